@@ -1,0 +1,217 @@
+/*
+ * ouranos_amd.h -- C ABI of the MI355X DG-Sn transport-sweep engine.
+ *
+ * The library replaces the CPU sweep behind the reference's
+ * RadiativeTransfer<dim,tensor_dim> Epetra_Operator (Rombur/ouranos,
+ * src/RadiativeTransfer.hh:46-127): the operator y = x - D2M L^-1 M2D S(x)
+ * that AztecOO GMRES/BiCGStab and source iteration call.  Plain pointers and
+ * sizes only; no C++ types cross this boundary.  Every entry point returns
+ * OU_OK (0) or a negative ou_status; ou_last_error() gives the message of the
+ * last failure on the calling thread.
+ *
+ * Vector layout (all FP64):  x[m * n_dofs + cell * N + l]
+ *   cell = i + nx*(j + ny*k),  l = lx + (p+1)*(ly + (p+1)*lz),  N = (p+1)^dim
+ */
+#ifndef OURANOS_AMD_H
+#define OURANOS_AMD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  OU_OK = 0,
+  OU_ERR_ARG = -1,         /* invalid argument or inconsistent descriptor */
+  OU_ERR_SIZE = -2,        /* vector / plan size mismatch */
+  OU_ERR_HIP = -3,         /* HIP runtime error */
+  OU_ERR_UNSUPPORTED = -4, /* valid request the library does not implement */
+  OU_ERR_NONFINITE = -5,   /* NaN/Inf detected in an input */
+  OU_ERR_NOT_CONVERGED = -6,
+  OU_ERR_NOMEM = -7,
+  OU_ERR_TIMEOUT = -8
+} ou_status;
+
+typedef enum { OU_FP64 = 0, OU_FP32 = 1 } ou_precision;
+typedef enum { OU_BC_VACUUM = 0, OU_BC_ISOTROPIC = 1, OU_BC_MOST_NORMAL = 2 } ou_bc;
+typedef enum { OU_QUAD_LS = 0, OU_QUAD_GLC = 1, OU_QUAD_PRODUCT_GLC = 2 } ou_quad_kind;
+typedef enum {
+  OU_SOLVER_SI = 0,
+  OU_SOLVER_GMRES = 1,
+  OU_SOLVER_BICGSTAB = 2,
+  OU_SOLVER_FGMRES = 3
+} ou_solver_kind;
+
+typedef struct ou_plan ou_plan;
+typedef struct ou_quadrature ou_quadrature;
+
+/* ------------------------------------------------------------------------ */
+/* Errors                                                                    */
+/* ------------------------------------------------------------------------ */
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *ou_last_error(void);
+
+/* ------------------------------------------------------------------------ */
+/* Angular quadrature and moment operators                                  */
+/* Replaces RTQuadrature / GLC / LS (src/RTQuadrature.hh:26-98,             */
+/* src/GLC.hh:21-41, src/LS.hh:19-30): ctor(sn, L_max, galerkin) +          */
+/* build_quadrature(weight_sum, dim) (src/RTQuadrature.cc:27-56).           */
+/* kind OU_QUAD_LS / OU_QUAD_GLC use `sn`; OU_QUAD_PRODUCT_GLC uses         */
+/* n_polar (Gauss-Legendre nodes on (-1,1)) x n_azim (Chebyshev azimuths). */
+/* dim 3 deploys all eight octants (the reference deploys four, defect A.1).*/
+/* ------------------------------------------------------------------------ */
+int ou_quadrature_create(int kind, int dim, int sn, int n_polar, int n_azim,
+                         int L_max, int galerkin, double weight_sum,
+                         ou_quadrature **quad);
+void ou_quadrature_destroy(ou_quadrature *quad);
+/* get_n_dir / get_n_mom (src/RTQuadrature.hh:100-108) */
+int ou_quadrature_n_dir(const ou_quadrature *quad);
+int ou_quadrature_n_mom(const ou_quadrature *quad);
+/* Copies get_omega (n_dir x 3), the weights (n_dir), get_M2D (n_dir x
+ * n_mom), get_D2M (n_mom x n_dir), get_l (n_mom) and is_most_normal_direction
+ * (2*dim x n_dir flags) out (src/RTQuadrature.hh:39-59,100-133).  Any
+ * output pointer may be NULL. */
+int ou_quadrature_copy(const ou_quadrature *quad, double *omega, double *weight,
+                       double *M2D, double *D2M, int *mom_l,
+                       unsigned char *most_normal);
+
+/* ------------------------------------------------------------------------ */
+/* Reference-element DG matrices                                            */
+/* Replaces FECell (src/FECell.hh:25-79, src/FECell.cc:10-78) on an axis-   */
+/* aligned cell of size h[0..dim): M (N x N), G (dim x N x N), E and U      */
+/* (2*dim x N x N), normals (2*dim x 3).  Faces ordered -x,+x,-y,+y,-z,+z.  */
+/* ------------------------------------------------------------------------ */
+int ou_fe_matrices(int dim, int fe_order, const double *h, double *M, double *G,
+                   double *E, double *U, double *normals);
+
+/* ------------------------------------------------------------------------ */
+/* Problem description and plan                                             */
+/* Replaces the RadiativeTransfer constructor + setup()                     */
+/* (src/RadiativeTransfer.hh:50-77, src/RadiativeTransfer.cc:49-78) on a    */
+/* structured Cartesian mesh.  All arrays are copied; the caller may free  */
+/* them after ou_plan_create returns.                                       */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+  int dim;               /* 2 or 3 */
+  int fe_order;          /* p: 1..3 (N = (p+1)^dim <= 64) */
+  int n_cells[3];        /* cells per axis (n_cells[2] ignored in 2D) */
+  double cell_size[3];   /* cell sizes */
+  int n_dir;
+  const double *omega;   /* n_dir x 3 */
+  int n_mom;             /* <= 16 */
+  const double *M2D;     /* n_dir x n_mom */
+  const double *D2M;     /* n_mom x n_dir */
+  const int *mom_l;      /* n_mom Legendre orders */
+  int n_mat;
+  int L;                 /* sigma_s holds orders 0..L per material */
+  const double *sigma_t; /* n_mat          (get_sigma_t, RTMaterialProperties.hh:84-90) */
+  const double *sigma_s; /* n_mat x (L+1)  (get_sigma_s, RTMaterialProperties.hh:92-101) */
+  const int *material_id;   /* n_cells */
+  const double *source;     /* n_cells isotropic source intensity, or NULL */
+  double weight_sum;        /* get_weight_sum (Parameters.hh:76) */
+  int bc[6];                /* ou_bc per face (get_bc_type, Parameters.hh:89) */
+  double inc_flux[6];       /* get_inc_flux (Parameters.hh:73) */
+  const unsigned char *most_normal; /* 2*dim x n_dir, or NULL */
+  int precision;            /* ou_precision of the sweep */
+} ou_problem_desc;
+
+int ou_plan_create(const ou_problem_desc *desc, ou_plan **plan);
+void ou_plan_destroy(ou_plan *plan);
+
+/* set_group (src/RadiativeTransfer.hh:73,251-255).  One group: g must be 0. */
+int ou_set_group(ou_plan *plan, int g);
+
+/* Sizes of the FP64 vectors the operator maps: n_mom * n_dofs
+ * (OperatorDomainMap / OperatorRangeMap, src/RadiativeTransfer.hh:120-126). */
+long ou_domain_size(const ou_plan *plan);
+long ou_range_size(const ou_plan *plan);
+long ou_n_dofs(const ou_plan *plan);
+
+/* Label() (src/RadiativeTransfer.hh:106): "radiative_transfer". */
+const char *ou_label(const ou_plan *plan);
+
+/* Apply (src/RadiativeTransfer.hh:94, src/RadiativeTransfer.cc:780-809):
+ * y = x - D2M L^-1 M2D S(x) on host vectors (copied in and out). */
+int ou_apply(ou_plan *plan, const double *x, double *y);
+
+/* Apply on device vectors of ou_domain_size doubles, enqueued on `stream`
+ * (a hipStream_t, or NULL for the plan's stream).  No host synchronisation. */
+int ou_apply_device(ou_plan *plan, const double *d_x, double *d_y, void *stream);
+
+/* compute_scattering_source (src/RadiativeTransfer.hh:90,
+ * src/RadiativeTransfer.cc:811-843): stores S(x) in the plan. */
+int ou_scatter_source(ou_plan *plan, const double *x);
+int ou_scatter_source_device(ou_plan *plan, const double *d_x, void *stream);
+
+/* The task loop initialize_scheduler/get_next_task/sweep/free_buffers
+ * (src/RadiativeTransfer.hh:59-87, src/RadiativeTransfer.cc:605-1012) as one
+ * call: y += D2M L^-1 (M2D S [+ M q/ws + inflow if with_source]), S from the
+ * last ou_scatter_source call (zero after ou_plan_create). */
+int ou_sweep(ou_plan *plan, double *y, int with_source);
+int ou_sweep_device(ou_plan *plan, double *d_y, int with_source, void *stream);
+
+/* SetUseTranspose / ApplyInverse / UseTranspose / HasNormInf / NormInf
+ * (src/RadiativeTransfer.hh:97-114).  The reference returns 0 without doing
+ * anything (and UseTranspose() returns true, defect A.8); these report
+ * OU_ERR_UNSUPPORTED / false / 0 explicitly. */
+int ou_set_use_transpose(ou_plan *plan, int use_transpose);
+int ou_apply_inverse(ou_plan *plan, const double *x, double *y);
+int ou_use_transpose(const ou_plan *plan);
+int ou_has_norm_inf(const ou_plan *plan);
+double ou_norm_inf(const ou_plan *plan);
+
+/* ------------------------------------------------------------------------ */
+/* Solver drivers (src/ouranos.cc:43-209: source iteration, GMRES and       */
+/* BiCGStab around the operator; FGMRES has no reference counterpart).      */
+/* Vectors stay in HBM; Arnoldi, inner products and the residual test are   */
+/* FP64.  phi holds the initial guess on entry, the solution on exit.       */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+  int solver;        /* ou_solver_kind */
+  double tol;        /* relative tolerance: ||b-Ax||/||b|| (SI: ||dphi||/||phi||) */
+  int max_it;        /* maximum operator applications */
+  int restart;       /* GMRES / FGMRES restart length (30 if 0) */
+  int verbose;
+} ou_solver_opts;
+
+typedef struct {
+  int iterations;      /* operator applications used */
+  double rel_residual; /* final relative residual (FP64 test) */
+  double seconds;      /* wall time of the solve */
+  int converged;
+} ou_solve_info;
+
+int ou_solve(ou_plan *plan, const ou_solver_opts *opts, double *phi,
+             ou_solve_info *info);
+
+/* ------------------------------------------------------------------------ */
+/* Device memory and timing plumbing (for callers without a HIP runtime of  */
+/* their own, e.g. ctypes).                                                 */
+/* ------------------------------------------------------------------------ */
+int ou_device_alloc(size_t bytes, void **ptr);
+int ou_device_free(void *ptr);
+int ou_memcpy_htod(void *dst, const void *src, size_t bytes);
+int ou_memcpy_dtoh(void *dst, const void *src, size_t bytes);
+int ou_device_synchronize(void);
+int ou_device_name(char *buf, int len);
+
+typedef struct {
+  double ms_total;        /* event time over all timed applications */
+  double ms_sweep_kernel; /* summed event time of the sweep-level launches */
+  double ms_other;        /* scatter source + trace init + epilogue */
+  int n_sweep_launches;   /* sweep-level launches per application */
+  int applications;
+} ou_timing;
+
+/* Runs `warmup` untimed then `steps` timed applications of the operator on
+ * device vectors, with HIP events on the plan's stream around each launch
+ * group.  Inputs are resident in HBM before the first event. */
+int ou_time_apply(ou_plan *plan, const double *d_x, double *d_y, int warmup,
+                  int steps, ou_timing *timing);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

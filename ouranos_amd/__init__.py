@@ -1,0 +1,356 @@
+"""MI355X DG-Sn transport-sweep engine: Python binding of the C ABI.
+
+The engine replaces the CPU sweep behind Ouranos' RadiativeTransfer operator
+(src/RadiativeTransfer.hh:46-127 in Rombur/ouranos).  This module mirrors the
+reference's classes for Python callers:
+
+  Quadrature          GLC / LS (RTQuadrature.hh:26-98) -> ou_quadrature_*
+  fe_matrices         FECell (FECell.hh:25-79)         -> ou_fe_matrices
+  RadiativeTransfer   the Epetra_Operator (Apply, Label, maps, sweep
+                      primitives, RadiativeTransfer.hh:46-127) -> ou_plan_*
+  solve               the drivers of ouranos.cc:43-209  -> ou_solve
+
+Every call goes through libouranos_amd.so (HIP kernels for gfx950).  There
+is no CPU fallback: if the library is missing, importing the binding fails.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+__all__ = [
+    "Quadrature", "RadiativeTransfer", "fe_matrices", "OuError", "lib", "LIB_PATH",
+    "FP64", "FP32", "LS", "GLC", "PRODUCT_GLC", "SI", "GMRES", "BICGSTAB", "FGMRES",
+    "BC_VACUUM", "BC_ISOTROPIC", "BC_MOST_NORMAL", "DeviceBuffer", "device_name",
+]
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libouranos_amd.so")
+
+FP64, FP32 = 0, 1
+LS, GLC, PRODUCT_GLC = 0, 1, 2
+SI, GMRES, BICGSTAB, FGMRES = 0, 1, 2, 3
+BC_VACUUM, BC_ISOTROPIC, BC_MOST_NORMAL = 0, 1, 2
+
+STATUS = {0: "OK", -1: "ARG", -2: "SIZE", -3: "HIP", -4: "UNSUPPORTED", -5: "NONFINITE",
+          -6: "NOT_CONVERGED", -7: "NOMEM", -8: "TIMEOUT"}
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_up = ctypes.POINTER(ctypes.c_ubyte)
+_vp = ctypes.c_void_p
+
+
+class OuError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"ouranos_amd error {code} ({STATUS.get(code, '?')}): {msg}")
+        self.code = code
+
+
+class ProblemDesc(ctypes.Structure):
+    _fields_ = [
+        ("dim", ctypes.c_int), ("fe_order", ctypes.c_int), ("n_cells", ctypes.c_int * 3),
+        ("cell_size", ctypes.c_double * 3), ("n_dir", ctypes.c_int), ("omega", _dp),
+        ("n_mom", ctypes.c_int), ("M2D", _dp), ("D2M", _dp), ("mom_l", _ip),
+        ("n_mat", ctypes.c_int), ("L", ctypes.c_int), ("sigma_t", _dp), ("sigma_s", _dp),
+        ("material_id", _ip), ("source", _dp), ("weight_sum", ctypes.c_double),
+        ("bc", ctypes.c_int * 6), ("inc_flux", ctypes.c_double * 6), ("most_normal", _up),
+        ("precision", ctypes.c_int),
+    ]
+
+
+class SolverOpts(ctypes.Structure):
+    _fields_ = [("solver", ctypes.c_int), ("tol", ctypes.c_double), ("max_it", ctypes.c_int),
+                ("restart", ctypes.c_int), ("verbose", ctypes.c_int)]
+
+
+class SolveInfo(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int), ("rel_residual", ctypes.c_double),
+                ("seconds", ctypes.c_double), ("converged", ctypes.c_int)]
+
+
+class Timing(ctypes.Structure):
+    _fields_ = [("ms_total", ctypes.c_double), ("ms_sweep_kernel", ctypes.c_double),
+                ("ms_other", ctypes.c_double), ("n_sweep_launches", ctypes.c_int),
+                ("applications", ctypes.c_int)]
+
+
+_lib = None
+
+
+def lib():
+    """Loads libouranos_amd.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is missing: build it with `python -m ouranos_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        L.ou_last_error.restype = ctypes.c_char_p
+        L.ou_label.restype = ctypes.c_char_p
+        L.ou_domain_size.restype = ctypes.c_long
+        L.ou_range_size.restype = ctypes.c_long
+        L.ou_n_dofs.restype = ctypes.c_long
+        L.ou_norm_inf.restype = ctypes.c_double
+        L.ou_plan_destroy.restype = None
+        L.ou_quadrature_destroy.restype = None
+        L.ou_device_alloc.argtypes = [ctypes.c_size_t, ctypes.POINTER(_vp)]
+        L.ou_device_free.argtypes = [_vp]
+        L.ou_memcpy_htod.argtypes = [_vp, _vp, ctypes.c_size_t]
+        L.ou_memcpy_dtoh.argtypes = [_vp, _vp, ctypes.c_size_t]
+        L.ou_apply_device.argtypes = [_vp, _vp, _vp, _vp]
+        L.ou_sweep_device.argtypes = [_vp, _vp, ctypes.c_int, _vp]
+        L.ou_scatter_source_device.argtypes = [_vp, _vp, _vp]
+        L.ou_time_apply.argtypes = [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int,
+                                    ctypes.POINTER(Timing)]
+        _lib = L
+    return _lib
+
+
+def _check(code):
+    if code != 0:
+        raise OuError(code, lib().ou_last_error().decode())
+    return code
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    return a.ctypes.data_as(_ip)
+
+
+def device_name():
+    buf = ctypes.create_string_buffer(256)
+    _check(lib().ou_device_name(buf, 256))
+    return buf.value.decode()
+
+
+class Quadrature:
+    """Angular quadrature and moment operators (RTQuadrature + GLC/LS).
+
+    kind LS/GLC take `sn`; PRODUCT_GLC takes n_polar x n_azim.  dim=3
+    deploys eight octants.  Arrays: omega (n_dir,3), weight (n_dir),
+    M2D (n_dir,n_mom), D2M (n_mom,n_dir), mom_l (n_mom),
+    most_normal (2*dim, n_dir).
+    """
+
+    def __init__(self, kind, dim, sn=0, n_polar=0, n_azim=0, L_max=0, galerkin=False,
+                 weight_sum=4.0 * np.pi):
+        h = ctypes.c_void_p()
+        _check(lib().ou_quadrature_create(int(kind), int(dim), int(sn), int(n_polar),
+                                          int(n_azim), int(L_max), int(bool(galerkin)),
+                                          ctypes.c_double(weight_sum), ctypes.byref(h)))
+        try:
+            nd = lib().ou_quadrature_n_dir(h)
+            nm = lib().ou_quadrature_n_mom(h)
+            self.omega = np.zeros((nd, 3))
+            self.weight = np.zeros(nd)
+            self.M2D = np.zeros((nd, nm))
+            self.D2M = np.zeros((nm, nd))
+            self.mom_l = np.zeros(nm, dtype=np.int32)
+            self.most_normal = np.zeros((2 * dim, nd), dtype=np.uint8)
+            _check(lib().ou_quadrature_copy(h, _d(self.omega), _d(self.weight), _d(self.M2D),
+                                            _d(self.D2M), _i(self.mom_l),
+                                            self.most_normal.ctypes.data_as(_up)))
+        finally:
+            lib().ou_quadrature_destroy(h)
+        self.dim, self.weight_sum = dim, weight_sum
+        self.n_dir, self.n_mom = nd, nm
+
+    def get_n_dir(self):
+        return self.n_dir
+
+    def get_n_mom(self):
+        return self.n_mom
+
+    def get_l(self, i):
+        return int(self.mom_l[i])
+
+    def get_omega(self, idir):
+        return self.omega[idir]
+
+    def is_most_normal_direction(self, face, idir):
+        return bool(self.most_normal[face, idir])
+
+
+def fe_matrices(dim, p, h):
+    """M, G (dim,N,N), E (2dim,N,N), U (2dim,N,N), normals (2dim,3) of one cell."""
+    N = (p + 1) ** dim
+    M = np.zeros((N, N))
+    G = np.zeros((dim, N, N))
+    E = np.zeros((2 * dim, N, N))
+    U = np.zeros((2 * dim, N, N))
+    nrm = np.zeros((2 * dim, 3))
+    hh = np.ascontiguousarray(list(h) + [1.0] * (3 - len(h)), dtype=np.float64)
+    _check(lib().ou_fe_matrices(dim, p, _d(hh), _d(M), _d(G), _d(E), _d(U), _d(nrm)))
+    return M, G, E, U, nrm
+
+
+class DeviceBuffer:
+    """Device memory owned through the library (no second HIP runtime needed)."""
+
+    def __init__(self, nbytes):
+        self.ptr = ctypes.c_void_p()
+        self.nbytes = int(nbytes)
+        _check(lib().ou_device_alloc(ctypes.c_size_t(self.nbytes), ctypes.byref(self.ptr)))
+
+    def upload(self, arr):
+        arr = np.ascontiguousarray(arr)
+        assert arr.nbytes <= self.nbytes
+        _check(lib().ou_memcpy_htod(self.ptr, arr.ctypes.data_as(_vp), arr.nbytes))
+
+    def download(self, arr):
+        assert arr.nbytes <= self.nbytes and arr.flags.c_contiguous
+        _check(lib().ou_memcpy_dtoh(arr.ctypes.data_as(_vp), self.ptr, arr.nbytes))
+        return arr
+
+    def free(self):
+        if self.ptr:
+            lib().ou_device_free(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class RadiativeTransfer:
+    """The transport operator y = x - D2M L^-1 M2D S(x) on the GPU.
+
+    Mirrors RadiativeTransfer<dim,tensor_dim> (src/RadiativeTransfer.hh:46-127):
+    Apply, Label, OperatorDomainMap/RangeMap sizes, set_group,
+    compute_scattering_source and the sweep.  Vectors are FP64 numpy arrays
+    of domain_size() entries in the moment-major layout x[m*n_dofs + dof].
+    """
+
+    def __init__(self, dim, fe_order, n_cells, cell_size, quad, sigma_t, sigma_s,
+                 material_id, source=None, weight_sum=None, bc=None, inc_flux=None,
+                 precision=FP64):
+        self._keep = []
+
+        def keep(a, dt):
+            a = np.ascontiguousarray(a, dtype=dt)
+            self._keep.append(a)
+            return a
+
+        ds = ProblemDesc()
+        ds.dim, ds.fe_order = int(dim), int(fe_order)
+        nc = list(n_cells) + [1] * (3 - len(n_cells))
+        hs = list(cell_size) + [1.0] * (3 - len(cell_size))
+        for a in range(3):
+            ds.n_cells[a] = int(nc[a])
+            ds.cell_size[a] = float(hs[a])
+        ds.n_dir = quad.n_dir
+        ds.omega = _d(keep(quad.omega, np.float64))
+        ds.n_mom = quad.n_mom
+        ds.M2D = _d(keep(quad.M2D, np.float64))
+        ds.D2M = _d(keep(quad.D2M, np.float64))
+        ds.mom_l = _i(keep(quad.mom_l, np.int32))
+        st = keep(sigma_t, np.float64).reshape(-1)
+        ss = keep(sigma_s, np.float64).reshape(st.shape[0], -1)
+        self._keep.append(ss)
+        ds.n_mat, ds.L = st.shape[0], ss.shape[1] - 1
+        ds.sigma_t, ds.sigma_s = _d(st), _d(ss)
+        ds.material_id = _i(keep(material_id, np.int32))
+        ds.source = _d(keep(source, np.float64)) if source is not None else None
+        ds.weight_sum = float(quad.weight_sum if weight_sum is None else weight_sum)
+        for f in range(6):
+            ds.bc[f] = int(bc[f]) if bc is not None and f < len(bc) else BC_VACUUM
+            ds.inc_flux[f] = float(inc_flux[f]) if inc_flux is not None and f < len(inc_flux) else 0.0
+        ds.most_normal = keep(quad.most_normal, np.uint8).ctypes.data_as(_up)
+        ds.precision = int(precision)
+        self._desc = ds
+        self._plan = ctypes.c_void_p()
+        _check(lib().ou_plan_create(ctypes.byref(ds), ctypes.byref(self._plan)))
+        self._keep = []
+        self.n = int(lib().ou_domain_size(self._plan))
+        self.n_dofs = int(lib().ou_n_dofs(self._plan))
+        self.n_mom = quad.n_mom
+
+    # --- Epetra_Operator surface -------------------------------------------
+    def Label(self):
+        return lib().ou_label(self._plan).decode()
+
+    def domain_size(self):
+        return self.n
+
+    def range_size(self):
+        return int(lib().ou_range_size(self._plan))
+
+    def Apply(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        if x.size != self.n:
+            raise OuError(-2, f"x has {x.size} entries, operator expects {self.n}")
+        y = np.empty(self.n)
+        _check(lib().ou_apply(self._plan, _d(x), _d(y)))
+        return y
+
+    def ApplyInverse(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.empty_like(x)
+        _check(lib().ou_apply_inverse(self._plan, _d(x), _d(y)))
+        return y
+
+    def SetUseTranspose(self, flag):
+        return _check(lib().ou_set_use_transpose(self._plan, int(bool(flag))))
+
+    def UseTranspose(self):
+        return bool(lib().ou_use_transpose(self._plan))
+
+    def HasNormInf(self):
+        return bool(lib().ou_has_norm_inf(self._plan))
+
+    def NormInf(self):
+        return float(lib().ou_norm_inf(self._plan))
+
+    # --- sweep primitives ------------------------------------------------------
+    def set_group(self, g):
+        _check(lib().ou_set_group(self._plan, int(g)))
+
+    def compute_scattering_source(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        _check(lib().ou_scatter_source(self._plan, _d(x)))
+
+    def sweep(self, y=None, with_source=False):
+        """y += D2M L^-1 (M2D S [+ source + inflow]); returns y."""
+        y = np.zeros(self.n) if y is None else np.ascontiguousarray(y, dtype=np.float64)
+        _check(lib().ou_sweep(self._plan, _d(y), int(bool(with_source))))
+        return y
+
+    # --- drivers ---------------------------------------------------------------
+    def solve(self, solver=GMRES, tol=1e-8, max_it=1000, restart=30, phi0=None,
+              verbose=False, raise_on_fail=True):
+        phi = np.zeros(self.n) if phi0 is None else np.array(phi0, dtype=np.float64)
+        opts = SolverOpts(int(solver), float(tol), int(max_it), int(restart), int(verbose))
+        info = SolveInfo()
+        code = lib().ou_solve(self._plan, ctypes.byref(opts), _d(phi), ctypes.byref(info))
+        if code != 0 and (raise_on_fail or code != -6):
+            _check(code)
+        return phi, {"iterations": info.iterations, "rel_residual": info.rel_residual,
+                     "seconds": info.seconds, "converged": bool(info.converged)}
+
+    # --- device API --------------------------------------------------------------
+    def apply_device(self, d_x, d_y):
+        _check(lib().ou_apply_device(self._plan, d_x.ptr, d_y.ptr, None))
+
+    def time_apply(self, d_x, d_y, warmup, steps):
+        t = Timing()
+        _check(lib().ou_time_apply(self._plan, d_x.ptr, d_y.ptr, int(warmup), int(steps),
+                                   ctypes.byref(t)))
+        return {"ms_total": t.ms_total, "ms_sweep_kernel": t.ms_sweep_kernel,
+                "ms_other": t.ms_other, "n_sweep_launches": t.n_sweep_launches,
+                "applications": t.applications}
+
+    def close(self):
+        if getattr(self, "_plan", None):
+            lib().ou_plan_destroy(self._plan)
+            self._plan = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,306 @@
+// Plan construction: validation of the problem descriptor, the wavefront
+// schedule and the device-resident data of the operator.
+//
+// The reference builds, per MPI rank, one FECell per cell and one Task per
+// (cell, direction) with hash maps of upwind dependencies
+// (RadiativeTransfer.cc:49-584, Task.cc).  On a structured Cartesian mesh the
+// dependency DAG of a direction is fixed by the signs of Omega: cell
+// (i,j,k) waits for its neighbours one step upwind along each axis, so the
+// cells with the same i'+j'+k' (coordinates reflected into the octant's
+// frame) form one level of independent tasks, shared by every direction of
+// the octant.  The schedule below lists, per level, chunks of at most 64
+// cells of one octant and one material.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+#include "fe.hpp"
+#include "plan.hpp"
+
+namespace ou {
+
+namespace {
+
+template <typename V>
+void upload(void *dst, const V &v) {
+  OU_HIP(hipMemcpy(dst, v.data(), v.size() * sizeof(typename V::value_type),
+                   hipMemcpyHostToDevice));
+}
+
+void *dev_alloc(size_t bytes) {
+  void *p = nullptr;
+  if (bytes == 0) bytes = 8;
+  hipError_t e = hipMalloc(&p, bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    char buf[160];
+    std::snprintf(buf, sizeof buf, "hipMalloc of %.3f GB failed", bytes / 1e9);
+    fail(OU_ERR_NOMEM, buf);
+  }
+  return p;
+}
+
+// Converts a double table to the sweep precision and uploads it.
+void *upload_real(const std::vector<double> &v, int precision) {
+  if (precision == OU_FP32) {
+    std::vector<float> f(v.begin(), v.end());
+    void *p = dev_alloc(f.size() * 4);
+    upload(p, f);
+    return p;
+  }
+  void *p = dev_alloc(v.size() * 8);
+  upload(p, v);
+  return p;
+}
+
+bool finite_all(const double *p, size_t n) {
+  for (size_t i = 0; i < n; ++i)
+    if (!std::isfinite(p[i])) return false;
+  return true;
+}
+
+}  // namespace
+
+Plan::~Plan() {
+  void *bufs[] = {d_ainv,   d_S,        d_S0,        d_m2d,     d_omega_abs, d_face_mass,
+                  d_d2m,    d_partial,  d_trace[0],  d_trace[1], d_trace[2], d_dir_oct,
+                  d_oct_dirs, d_oct_off, d_chunks,  d_cells,   d_mat,       d_mom_l,
+                  d_sigma_s, d_source,  d_mass,      d_rowsum,  d_inflow,    d_x,
+                  d_y};
+  for (void *b : bufs)
+    if (b) (void)hipFree(b);
+  for (hipEvent_t e : ev)
+    if (e) (void)hipEventDestroy(e);
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+Plan *create_plan(const ou_problem_desc *desc) {
+  if (!desc) fail(OU_ERR_ARG, "null problem descriptor");
+  const ou_problem_desc &ds = *desc;
+  if (ds.dim != 2 && ds.dim != 3) fail(OU_ERR_ARG, "dim must be 2 or 3");
+  if (ds.fe_order < 1 || ds.fe_order > 5) fail(OU_ERR_ARG, "fe_order must be in 1..5");
+  if (ds.precision != OU_FP64 && ds.precision != OU_FP32)
+    fail(OU_ERR_ARG, "precision must be OU_FP64 or OU_FP32");
+  for (int a = 0; a < ds.dim; ++a) {
+    if (ds.n_cells[a] < 1) fail(OU_ERR_ARG, "n_cells must be >= 1 on every axis");
+    if (!(ds.cell_size[a] > 0.0) || !std::isfinite(ds.cell_size[a]))
+      fail(OU_ERR_ARG, "cell sizes must be positive and finite");
+  }
+  if (ds.n_dir < 1 || !ds.omega || !ds.M2D || !ds.D2M || !ds.mom_l)
+    fail(OU_ERR_ARG, "quadrature arrays missing");
+  if (ds.n_mom < 1 || ds.n_mom > 16) fail(OU_ERR_UNSUPPORTED, "n_mom must be in 1..16");
+  if (ds.n_mat < 1 || !ds.sigma_t || !ds.sigma_s || !ds.material_id)
+    fail(OU_ERR_ARG, "material arrays missing");
+  if (ds.L < 0) fail(OU_ERR_ARG, "L must be >= 0");
+  if (!(ds.weight_sum > 0.0)) fail(OU_ERR_ARG, "weight_sum must be positive");
+
+  auto *pl = new Plan();
+  try {
+    Plan &p = *pl;
+    p.dim = ds.dim;
+    p.p = ds.fe_order;
+    p.P1 = p.p + 1;
+    p.N = p.dim == 2 ? p.P1 * p.P1 : p.P1 * p.P1 * p.P1;
+    p.F = p.dim == 2 ? p.P1 : p.P1 * p.P1;
+    p.n[0] = ds.n_cells[0];
+    p.n[1] = ds.n_cells[1];
+    p.n[2] = p.dim == 3 ? ds.n_cells[2] : 1;
+    for (int a = 0; a < 3; ++a) p.h[a] = a < p.dim ? ds.cell_size[a] : 1.0;
+    p.n_dir = ds.n_dir;
+    p.n_mom = ds.n_mom;
+    p.n_mat = ds.n_mat;
+    p.L = ds.L;
+    p.precision = ds.precision;
+    p.n_cells = long(p.n[0]) * p.n[1] * p.n[2];
+    p.n_dofs = p.n_cells * p.N;
+    p.vec_size = p.n_dofs * p.n_mom;
+    p.ws = ds.weight_sum;
+    p.omega.assign(ds.omega, ds.omega + 3 * size_t(p.n_dir));
+    p.M2D.assign(ds.M2D, ds.M2D + size_t(p.n_dir) * p.n_mom);
+    p.D2M.assign(ds.D2M, ds.D2M + size_t(p.n_mom) * p.n_dir);
+    p.mom_l.assign(ds.mom_l, ds.mom_l + p.n_mom);
+    p.sigma_t.assign(ds.sigma_t, ds.sigma_t + p.n_mat);
+    p.sigma_s.assign(ds.sigma_s, ds.sigma_s + size_t(p.n_mat) * (p.L + 1));
+    p.mat.assign(ds.material_id, ds.material_id + p.n_cells);
+    if (ds.source)
+      p.source.assign(ds.source, ds.source + p.n_cells);
+    else
+      p.source.assign(p.n_cells, 0.0);
+    if (ds.most_normal)
+      p.most_normal.assign(ds.most_normal, ds.most_normal + size_t(2 * p.dim) * p.n_dir);
+    for (int f = 0; f < 6; ++f) {
+      p.bc[f] = f < 2 * p.dim ? ds.bc[f] : OU_BC_VACUUM;
+      p.inc[f] = f < 2 * p.dim ? ds.inc_flux[f] : 0.0;
+      if (p.bc[f] < OU_BC_VACUUM || p.bc[f] > OU_BC_MOST_NORMAL)
+        fail(OU_ERR_UNSUPPORTED, "boundary condition must be vacuum, isotropic or most normal");
+      if (p.bc[f] == OU_BC_MOST_NORMAL && !ds.most_normal)
+        fail(OU_ERR_ARG, "most-normal boundary needs the most_normal flags");
+    }
+    if (!finite_all(p.omega.data(), p.omega.size()) ||
+        !finite_all(p.M2D.data(), p.M2D.size()) ||
+        !finite_all(p.D2M.data(), p.D2M.size()) ||
+        !finite_all(p.sigma_t.data(), p.sigma_t.size()) ||
+        !finite_all(p.sigma_s.data(), p.sigma_s.size()) ||
+        !finite_all(p.source.data(), p.source.size()))
+      fail(OU_ERR_NONFINITE, "non-finite value in the problem descriptor");
+    for (long c = 0; c < p.n_cells; ++c)
+      if (p.mat[c] < 0 || p.mat[c] >= p.n_mat) fail(OU_ERR_ARG, "material id out of range");
+    for (int m = 0; m < p.n_mom; ++m)
+      if (p.mom_l[m] < 0) fail(OU_ERR_ARG, "negative moment order");
+    // Moment 0 must be the isotropic harmonic (constant over directions):
+    // the volumetric source is folded into it.
+    for (int d = 1; d < p.n_dir; ++d)
+      if (std::fabs(p.M2D[size_t(d) * p.n_mom] - p.M2D[0]) > 1e-12 * std::fabs(p.M2D[0]))
+        fail(OU_ERR_ARG, "moment 0 of M2D must be the isotropic harmonic");
+    if (p.M2D[0] == 0.0) fail(OU_ERR_ARG, "moment 0 of M2D is zero");
+
+    p.mm = p.n_mom <= 1 ? 1 : p.n_mom <= 4 ? 4 : p.n_mom <= 10 ? 10 : 16;
+    if (!sweep_supported(p.dim, p.p, p.mm, p.precision))
+      fail(OU_ERR_UNSUPPORTED,
+           "no GPU sweep kernel for this (dim, fe_order, n_mom): supported are "
+           "2D Q1-Q3 and 3D Q1-Q2 with up to 16 moments");
+
+    // ---- directions per octant -------------------------------------------
+    p.n_oct = p.dim == 2 ? 4 : 8;
+    p.dir_oct.resize(p.n_dir);
+    std::vector<double> omega_abs(3 * size_t(p.n_dir));
+    for (int d = 0; d < p.n_dir; ++d) {
+      int o = 0;
+      for (int a = 0; a < p.dim; ++a) {
+        if (p.omega[3 * d + a] < 0.0) o |= 1 << a;
+        omega_abs[3 * d + a] = std::fabs(p.omega[3 * d + a]);
+      }
+      p.dir_oct[d] = o;
+    }
+    p.oct_off.assign(p.n_oct + 1, 0);
+    for (int d = 0; d < p.n_dir; ++d) p.oct_off[p.dir_oct[d] + 1]++;
+    for (int o = 0; o < p.n_oct; ++o) p.oct_off[o + 1] += p.oct_off[o];
+    p.oct_dirs.resize(p.n_dir);
+    {
+      std::vector<int> fill(p.oct_off.begin(), p.oct_off.end() - 1);
+      for (int d = 0; d < p.n_dir; ++d) p.oct_dirs[fill[p.dir_oct[d]]++] = d;
+    }
+
+    // ---- wavefront schedule ------------------------------------------------
+    const int nx = p.n[0], ny = p.n[1], nz = p.n[2];
+    p.n_levels = nx + ny + nz - 2;
+    std::vector<std::vector<Chunk>> per_level(p.n_levels);
+    std::vector<int> cells;
+    cells.reserve(size_t(p.n_cells) * p.n_oct);
+    std::vector<std::vector<long>> bucket(p.n_levels);
+    for (int o = 0; o < p.n_oct; ++o) {
+      if (p.oct_off[o + 1] == p.oct_off[o]) continue;
+      for (auto &b : bucket) b.clear();
+      // Canonical coordinates (kk, jj, ii) in ascending order: within a
+      // level, consecutive cells then share kk and have consecutive jj.
+      for (int kk = 0; kk < nz; ++kk)
+        for (int jj = 0; jj < ny; ++jj)
+          for (int ii = 0; ii < nx; ++ii) {
+            const int i = (o & 1) ? nx - 1 - ii : ii;
+            const int j = (o & 2) ? ny - 1 - jj : jj;
+            const int k = (o & 4) ? nz - 1 - kk : kk;
+            bucket[ii + jj + kk].push_back(i + long(nx) * (j + long(ny) * k));
+          }
+      for (int l = 0; l < p.n_levels; ++l) {
+        std::vector<long> &b = bucket[l];
+        std::stable_sort(b.begin(), b.end(),
+                         [&](long a, long c) { return p.mat[a] < p.mat[c]; });
+        size_t s = 0;
+        while (s < b.size()) {
+          const int m = p.mat[b[s]];
+          size_t e = s;
+          while (e < b.size() && e - s < 64 && p.mat[b[e]] == m) ++e;
+          per_level[l].push_back(Chunk{o, m, int(e - s), int(cells.size())});
+          for (size_t q = s; q < e; ++q) cells.push_back(int(b[q]));
+          s = e;
+        }
+      }
+    }
+    std::vector<Chunk> chunks;
+    p.level_chunk_off.assign(p.n_levels + 1, 0);
+    for (int l = 0; l < p.n_levels; ++l) {
+      p.level_chunk_off[l] = int(chunks.size());
+      chunks.insert(chunks.end(), per_level[l].begin(), per_level[l].end());
+    }
+    p.level_chunk_off[p.n_levels] = int(chunks.size());
+    p.n_chunks = long(chunks.size());
+
+    // ---- boundary inflow per (direction, axis) ----------------------------
+    std::vector<double> inflow(3 * size_t(p.n_dir), 0.0);
+    for (int d = 0; d < p.n_dir; ++d)
+      for (int a = 0; a < p.dim; ++a) {
+        const int f = 2 * a + (p.omega[3 * d + a] >= 0.0 ? 0 : 1);
+        double v = 0.0;
+        if (p.bc[f] == OU_BC_ISOTROPIC ||
+            (p.bc[f] == OU_BC_MOST_NORMAL && p.most_normal[size_t(f) * p.n_dir + d]))
+          v = p.inc[f];
+        inflow[3 * d + a] = v / p.ws;
+      }
+
+    // ---- element data -------------------------------------------------------
+    RefElement re(p.dim, p.p, p.h);
+    std::vector<double> mass, fm, face_mass(3 * size_t(p.F) * p.F, 0.0);
+    re.mass(mass);
+    std::vector<double> rowsum(p.N, 0.0);
+    for (int i = 0; i < p.N; ++i)
+      for (int j = 0; j < p.N; ++j) rowsum[i] += mass[size_t(i) * p.N + j];
+    for (int a = 0; a < p.dim; ++a) {
+      re.face_mass(a, fm);
+      std::copy(fm.begin(), fm.end(), face_mass.begin() + size_t(a) * p.F * p.F);
+    }
+
+    // ---- device buffers -----------------------------------------------------
+    OU_HIP(hipStreamCreateWithFlags(&p.stream, hipStreamNonBlocking));
+    for (auto &e : p.ev) OU_HIP(hipEventCreate(&e));
+    const size_t rs = p.real_size();
+    p.d_ainv = dev_alloc(size_t(p.n_mat) * p.n_dir * p.N * p.N * rs);
+    p.d_S = dev_alloc(size_t(p.vec_size) * rs);
+    OU_HIP(hipMemset(p.d_S, 0, size_t(p.vec_size) * rs));
+    p.d_S0 = dev_alloc(size_t(p.n_dofs) * rs);
+    p.d_m2d = upload_real(p.M2D, p.precision);
+    p.d_omega_abs = upload_real(omega_abs, p.precision);
+    p.d_face_mass = upload_real(face_mass, p.precision);
+    p.d_d2m = static_cast<double *>(dev_alloc(p.D2M.size() * 8));
+    upload(p.d_d2m, p.D2M);
+    p.d_partial = static_cast<double *>(dev_alloc(size_t(p.n_oct) * p.vec_size * 8));
+    OU_HIP(hipMemset(p.d_partial, 0, size_t(p.n_oct) * p.vec_size * 8));
+    p.plane_size[0] = long(ny) * nz;
+    p.plane_size[1] = long(nx) * nz;
+    p.plane_size[2] = long(nx) * ny;
+    for (int a = 0; a < p.dim; ++a)
+      p.d_trace[a] = dev_alloc(size_t(p.n_dir) * p.F * p.plane_size[a] * rs);
+    p.d_dir_oct = static_cast<int *>(dev_alloc(p.dir_oct.size() * 4));
+    upload(p.d_dir_oct, p.dir_oct);
+    p.d_oct_dirs = static_cast<int *>(dev_alloc(p.oct_dirs.size() * 4));
+    upload(p.d_oct_dirs, p.oct_dirs);
+    p.d_oct_off = static_cast<int *>(dev_alloc(p.oct_off.size() * 4));
+    upload(p.d_oct_off, p.oct_off);
+    p.d_chunks = static_cast<Chunk *>(dev_alloc(chunks.size() * sizeof(Chunk)));
+    upload(p.d_chunks, chunks);
+    p.d_cells = static_cast<int *>(dev_alloc(cells.size() * 4));
+    upload(p.d_cells, cells);
+    p.d_mat = static_cast<int *>(dev_alloc(p.mat.size() * 4));
+    upload(p.d_mat, p.mat);
+    p.d_mom_l = static_cast<int *>(dev_alloc(p.mom_l.size() * 4));
+    upload(p.d_mom_l, p.mom_l);
+    p.d_sigma_s = static_cast<double *>(dev_alloc(p.sigma_s.size() * 8));
+    upload(p.d_sigma_s, p.sigma_s);
+    p.d_source = static_cast<double *>(dev_alloc(p.source.size() * 8));
+    upload(p.d_source, p.source);
+    p.d_mass = static_cast<double *>(dev_alloc(mass.size() * 8));
+    upload(p.d_mass, mass);
+    p.d_rowsum = static_cast<double *>(dev_alloc(rowsum.size() * 8));
+    upload(p.d_rowsum, rowsum);
+    p.d_inflow = static_cast<double *>(dev_alloc(inflow.size() * 8));
+    upload(p.d_inflow, inflow);
+
+    launch_build_inverses(p);
+    return pl;
+  } catch (...) {
+    delete pl;
+    throw;
+  }
+}
+
+}  // namespace ou

@@ -1,0 +1,104 @@
+// The sweep plan: everything ou_plan_create builds once per problem, and
+// the launch sequence of one operator application.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "ou_common.hpp"
+
+namespace ou {
+
+// One chunk: up to 64 cells of one sweep level of one octant, all with the
+// same material (so every lane of a wave uses the same inverse matrix).
+struct Chunk {
+  int octant;
+  int material;
+  int count;
+  int cell_offset;  // into the cell list
+};
+
+// Device-side views handed to the kernels (plain pointers, by value).
+struct SweepView {
+  int n[3];
+  int n_dir, n_mom;
+  long n_dofs;
+  const void *ainv;        // [mat][dir][N][N] in the sweep precision
+  const void *S;           // [m][dof] scattering source (sweep precision)
+  const void *S0;          // moment-0 override for the source sweep, or null
+  const void *m2d;         // [dir][n_mom] (sweep precision)
+  const double *d2m;       // [m][dir]
+  const void *omega_abs;   // [dir][3] |Omega_a| (sweep precision)
+  const int *dir_oct;      // [dir] octant id
+  const int *oct_dirs;     // directions grouped by octant
+  const int *oct_off;      // [n_oct+1]
+  const void *face_mass;   // [axis][F][F] (sweep precision)
+  void *trace[3];          // per axis: [dir][F][plane]
+  long plane_size[3];
+  double *partial;         // [oct][m][dof]
+  const Chunk *chunks;
+  const int *cells;
+};
+
+struct Plan {
+  // problem
+  int dim = 2, p = 1, P1 = 2, N = 4, F = 2;
+  int n[3] = {1, 1, 1};
+  double h[3] = {1, 1, 1};
+  int n_dir = 0, n_mom = 0, n_mat = 0, L = 0, precision = OU_FP64;
+  long n_cells = 0, n_dofs = 0, vec_size = 0;
+  double ws = 1.0;
+  std::vector<double> omega, M2D, D2M, sigma_t, sigma_s, source;
+  std::vector<int> mom_l, mat;
+  std::vector<unsigned char> most_normal;
+  int bc[6] = {0, 0, 0, 0, 0, 0};
+  double inc[6] = {0, 0, 0, 0, 0, 0};
+  int group = 0;
+
+  // schedule
+  int n_oct = 4, n_levels = 0, waves = 4, mm = 1;
+  std::vector<int> oct_dirs, oct_off, dir_oct;
+  std::vector<int> level_chunk_off;  // [n_levels+1]
+  long n_chunks = 0;
+
+  // device buffers
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  void *d_ainv = nullptr, *d_S = nullptr, *d_S0 = nullptr, *d_m2d = nullptr;
+  void *d_omega_abs = nullptr, *d_face_mass = nullptr;
+  double *d_d2m = nullptr, *d_partial = nullptr;
+  void *d_trace[3] = {nullptr, nullptr, nullptr};
+  long plane_size[3] = {0, 0, 0};
+  int *d_dir_oct = nullptr, *d_oct_dirs = nullptr, *d_oct_off = nullptr;
+  Chunk *d_chunks = nullptr;
+  int *d_cells = nullptr, *d_mat = nullptr, *d_mom_l = nullptr;
+  double *d_sigma_s = nullptr, *d_source = nullptr, *d_mass = nullptr;
+  double *d_rowsum = nullptr;
+  double *d_inflow = nullptr;  // [dir][3] inflow value on the upwind face of each axis
+  double *d_x = nullptr, *d_y = nullptr;  // staging for the host-pointer API
+  bool S_is_zero = true;
+
+  ~Plan();
+  size_t real_size() const { return precision == OU_FP32 ? 4 : 8; }
+  SweepView view() const;
+};
+
+Plan *create_plan(const ou_problem_desc *desc);
+bool sweep_supported(int dim, int p, int mm, int precision);
+
+// Launch sequences (stream-ordered, no host synchronisation).
+void launch_build_inverses(Plan &pl);
+void launch_scatter_source(Plan &pl, const double *d_x, hipStream_t s);
+void launch_source_moment0(Plan &pl, hipStream_t s);
+void launch_trace_init(Plan &pl, bool with_source, hipStream_t s);
+void launch_sweep_levels(Plan &pl, const void *S0, hipStream_t s,
+                         hipEvent_t before = nullptr, hipEvent_t after = nullptr);
+void launch_epilogue(Plan &pl, const double *d_x, double *d_y, bool apply,
+                     hipStream_t s);
+
+// Whole operations.
+void apply_device(Plan &pl, const double *d_x, double *d_y, hipStream_t s);
+void sweep_device(Plan &pl, double *d_y, bool with_source, hipStream_t s);
+
+}  // namespace ou

@@ -1,0 +1,237 @@
+// Solver drivers around the transport operator (src/ouranos.cc:43-209):
+// source iteration, restarted GMRES, BiCGStab and flexible GMRES.  Vectors
+// live in HBM; the Arnoldi process, every inner product and the residual test
+// are FP64.  Small dense work (Hessenberg, Givens rotations) is on the host.
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <vector>
+
+#include "radiative_transfer.hpp"
+#include "vec.hpp"
+
+namespace ou {
+
+namespace {
+
+struct DevBuf {
+  double *p = nullptr;
+  explicit DevBuf(size_t n) {
+    if (hipMalloc(&p, (n ? n : 1) * sizeof(double)) != hipSuccess) {
+      (void)hipGetLastError();
+      fail(OU_ERR_NOMEM, "solver workspace allocation failed");
+    }
+  }
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+};
+
+// Dot products <V_i, w>, i < m, returned on the host.
+struct Dotter {
+  DevBuf scratch, out;
+  std::vector<double> host;
+  explicit Dotter(int max_m) : scratch(size_t(reduce_scratch_doubles(max_m))), out(size_t(max_m)), host(max_m) {}
+  const std::vector<double> &dots(long n, int m, const double *V, long ldv, const double *w,
+                                  hipStream_t s) {
+    vec_multi_dot(n, m, V, ldv, w, scratch.p, out.p, s);
+    OU_HIP(hipMemcpyAsync(host.data(), out.p, size_t(m) * 8, hipMemcpyDeviceToHost, s));
+    OU_HIP(hipStreamSynchronize(s));
+    return host;
+  }
+  double dot(long n, const double *a, const double *b, hipStream_t s) {
+    return dots(n, 1, a, 0, b, s)[0];
+  }
+  double norm(long n, const double *a, hipStream_t s) { return std::sqrt(dot(n, a, a, s)); }
+};
+
+}  // namespace
+
+void build_rhs(RadiativeTransfer &rt, double *d_rhs, hipStream_t s) {
+  const long n = rt.DomainSize();
+  rt.clear_scattering_source(s);
+  vec_fill(n, 0.0, d_rhs, s);
+  rt.sweep(d_rhs, true, s);
+}
+
+SolveResult solve_si(RadiativeTransfer &rt, double *d_phi, double tol, int max_it,
+                     bool verbose) {
+  hipStream_t s = rt.plan().stream;
+  const long n = rt.DomainSize();
+  DevBuf nw(n);
+  Dotter dt(1);
+  SolveResult res;
+  for (int it = 0; it < max_it; ++it) {
+    rt.compute_scattering_source(d_phi, s);
+    vec_fill(n, 0.0, nw.p, s);
+    rt.sweep(nw.p, true, s);
+    vec_axpby(n, 1.0, nw.p, -1.0, d_phi, s);  // d_phi <- new - old
+    const double num = dt.norm(n, d_phi, s);
+    const double den = dt.norm(n, nw.p, s);
+    vec_copy(n, nw.p, d_phi, s);
+    res.iterations = it + 1;
+    res.rel_residual = den > 0.0 ? num / den : 0.0;
+    if (verbose) std::printf("SI iteration %d: %.3e\n", it, res.rel_residual);
+    if (res.rel_residual < tol) {
+      res.converged = true;
+      break;
+    }
+  }
+  OU_HIP(hipStreamSynchronize(s));
+  return res;
+}
+
+SolveResult solve_gmres(Operator &op, const double *d_b, double *d_x, int restart,
+                        double tol, int max_it, Preconditioner *prec, bool verbose,
+                        hipStream_t s) {
+  const long n = op.DomainSize();
+  const int m = restart > 0 ? restart : 30;
+  if (m > 31) fail(OU_ERR_ARG, "restart length must be <= 31");
+  DevBuf V(size_t(n) * (m + 1)), w(n), hd(size_t(m + 1));
+  std::unique_ptr<DevBuf> Z;
+  if (prec) Z.reset(new DevBuf(size_t(n) * m));
+  Dotter dt(m + 1);
+  std::vector<double> H(size_t(m + 1) * m, 0.0), cs(m), sn(m), g(m + 1), y(m), h(m + 1);
+  SolveResult res;
+  const double bnorm = dt.norm(n, d_b, s);
+  if (bnorm == 0.0) {
+    vec_fill(n, 0.0, d_x, s);
+    res.converged = true;
+    OU_HIP(hipStreamSynchronize(s));
+    return res;
+  }
+  int total = 0;
+  while (true) {
+    // r = b - A x
+    op.Apply(d_x, V.p, s);
+    vec_lincomb(n, 1.0, d_b, -1.0, V.p, V.p, s);
+    const double beta = dt.norm(n, V.p, s);
+    res.rel_residual = beta / bnorm;
+    if (verbose) std::printf("GMRES restart at %d: true residual %.3e\n", total, res.rel_residual);
+    if (res.rel_residual < tol) {
+      res.converged = true;
+      break;
+    }
+    if (total >= max_it) break;
+    vec_scale(n, 1.0 / beta, V.p, s);
+    std::fill(g.begin(), g.end(), 0.0);
+    g[0] = beta;
+    int j = 0;
+    for (; j < m && total < max_it; ++j) {
+      double *vj = V.p + size_t(j) * n;
+      if (prec) {
+        double *zj = Z->p + size_t(j) * n;
+        prec->apply(vj, zj, s);
+        op.Apply(zj, w.p, s);
+      } else {
+        op.Apply(vj, w.p, s);
+      }
+      ++total;
+      // Classical Gram-Schmidt, applied twice (FP64).
+      for (int i = 0; i <= j; ++i) h[i] = 0.0;
+      for (int pass = 0; pass < 2; ++pass) {
+        const std::vector<double> &hh = dt.dots(n, j + 1, V.p, n, w.p, s);
+        for (int i = 0; i <= j; ++i) h[i] += hh[i];
+        OU_HIP(hipMemcpy(hd.p, hh.data(), size_t(j + 1) * 8, hipMemcpyHostToDevice));
+        vec_multi_axpy(n, j + 1, V.p, n, hd.p, 1.0, w.p, s);
+      }
+      const double hn = dt.norm(n, w.p, s);
+      for (int i = 0; i <= j; ++i) H[size_t(i) * m + j] = h[i];
+      H[size_t(j + 1) * m + j] = hn;
+      if (hn > 0.0) {
+        vec_copy(n, w.p, V.p + size_t(j + 1) * n, s);
+        vec_scale(n, 1.0 / hn, V.p + size_t(j + 1) * n, s);
+      }
+      for (int i = 0; i < j; ++i) {
+        const double a = H[size_t(i) * m + j], b = H[size_t(i + 1) * m + j];
+        H[size_t(i) * m + j] = cs[i] * a + sn[i] * b;
+        H[size_t(i + 1) * m + j] = -sn[i] * a + cs[i] * b;
+      }
+      const double a = H[size_t(j) * m + j], b = H[size_t(j + 1) * m + j];
+      const double r = std::hypot(a, b);
+      cs[j] = a / r;
+      sn[j] = b / r;
+      H[size_t(j) * m + j] = r;
+      H[size_t(j + 1) * m + j] = 0.0;
+      g[j + 1] = -sn[j] * g[j];
+      g[j] = cs[j] * g[j];
+      res.rel_residual = std::fabs(g[j + 1]) / bnorm;
+      if (verbose) std::printf("  it %d: %.3e\n", total, res.rel_residual);
+      if (res.rel_residual < tol || hn == 0.0) {
+        ++j;
+        break;
+      }
+    }
+    for (int i = j - 1; i >= 0; --i) {
+      double acc = g[i];
+      for (int k = i + 1; k < j; ++k) acc -= H[size_t(i) * m + k] * y[k];
+      y[i] = acc / H[size_t(i) * m + i];
+    }
+    // x += V y  (or Z y): multi_axpy subtracts, so pass -y.
+    for (int i = 0; i < j; ++i) h[i] = -y[i];
+    OU_HIP(hipStreamSynchronize(s));
+    OU_HIP(hipMemcpy(hd.p, h.data(), size_t(j) * 8, hipMemcpyHostToDevice));
+    vec_multi_axpy(n, j, prec ? Z->p : V.p, n, hd.p, 1.0, d_x, s);
+    OU_HIP(hipStreamSynchronize(s));
+  }
+  res.iterations = total;
+  OU_HIP(hipStreamSynchronize(s));
+  return res;
+}
+
+SolveResult solve_bicgstab(Operator &op, const double *d_b, double *d_x, double tol,
+                           int max_it, bool verbose, hipStream_t s) {
+  const long n = op.DomainSize();
+  DevBuf r(n), r0(n), p(n), v(n), sv(n), t(n);
+  Dotter dt(2);
+  SolveResult res;
+  const double bnorm = dt.norm(n, d_b, s);
+  if (bnorm == 0.0) {
+    vec_fill(n, 0.0, d_x, s);
+    res.converged = true;
+    OU_HIP(hipStreamSynchronize(s));
+    return res;
+  }
+  op.Apply(d_x, r.p, s);
+  vec_lincomb(n, 1.0, d_b, -1.0, r.p, r.p, s);
+  vec_copy(n, r.p, r0.p, s);
+  vec_fill(n, 0.0, p.p, s);
+  vec_fill(n, 0.0, v.p, s);
+  double rho = 1.0, alpha = 1.0, omega = 1.0;
+  res.rel_residual = dt.norm(n, r.p, s) / bnorm;
+  int it = 0;
+  while (res.rel_residual >= tol && it < max_it) {
+    ++it;
+    const double rho1 = dt.dot(n, r0.p, r.p, s);
+    const double beta = (rho1 / rho) * (alpha / omega);
+    rho = rho1;
+    // p = r + beta (p - omega v)
+    vec_axpby(n, -omega, v.p, 1.0, p.p, s);
+    vec_axpby(n, 1.0, r.p, beta, p.p, s);
+    op.Apply(p.p, v.p, s);
+    alpha = rho / dt.dot(n, r0.p, v.p, s);
+    vec_lincomb(n, 1.0, r.p, -alpha, v.p, sv.p, s);
+    const double snorm = dt.norm(n, sv.p, s);
+    if (snorm / bnorm < tol) {
+      vec_axpby(n, alpha, p.p, 1.0, d_x, s);
+      res.rel_residual = snorm / bnorm;
+      break;
+    }
+    op.Apply(sv.p, t.p, s);
+    const double ts = dt.dot(n, t.p, sv.p, s), tt = dt.dot(n, t.p, t.p, s);
+    omega = ts / tt;
+    vec_axpby(n, alpha, p.p, 1.0, d_x, s);
+    vec_axpby(n, omega, sv.p, 1.0, d_x, s);
+    vec_lincomb(n, 1.0, sv.p, -omega, t.p, r.p, s);
+    res.rel_residual = dt.norm(n, r.p, s) / bnorm;
+    if (verbose) std::printf("BiCGStab it %d: %.3e\n", it, res.rel_residual);
+  }
+  res.iterations = it;
+  res.converged = res.rel_residual < tol;
+  OU_HIP(hipStreamSynchronize(s));
+  return res;
+}
+
+}  // namespace ou
