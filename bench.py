@@ -1,0 +1,159 @@
+"""Benchmark of the transport sweep (the operator application of
+RadiativeTransfer::Apply) on one MI355X.
+
+One step = one application y = x - D2M L^-1 M2D S(x) of the full problem:
+scattering source, every sweep level of all octants, epilogue.  Inputs are
+resident in HBM before the timed region.  Default workload: 3D 128^3 hex
+mesh, DG Q2 (27 dofs/cell), Gauss-Legendre-Chebyshev 12x24 = 288 directions,
+L=1 anisotropic scattering, seeded 16^3-cell material blocks; FP32 angular
+flux with FP64 moment accumulation.
+
+Prints ONE JSON line.  `--gpus N` with N > 1 exits: this build runs on one GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import threading
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "sweep DoF-angle updates/s; time-to-1e-8 learned-FGMRES vs plain GMRES"
+HBM_PEAK_GBS = 8000.0
+
+
+def workload(name):
+    from ouranos_amd import problems as P
+    if name == "headline_3d":
+        return P.headline_3d(), ("3D 128^3 hex mesh, DG Q2 (27 dofs/cell), Gauss-Legendre-"
+                                 "Chebyshev 12 polar x 24 azimuthal = 288 directions, 1 group, "
+                                 "L=1 Henyey-Greenstein scattering, seeded 16^3-cell blocks "
+                                 "(sigma_t log-U[1,500], c U[0.99,0.9999]), vacuum")
+    if name == "hetero_2d":
+        return P.hetero_2d(), ("2D 256^2 quads, DG Q2, GLC 16x32 (256 directions, mu>0), L=3, "
+                               "seeded 16^2-cell blocks, vacuum")
+    if name == "uniform_2d":
+        return P.uniform_2d(), "2D 32^2 quads, DG Q1, LS-S8 (40 directions), c=0.5, q=1, vacuum"
+    if name == "thick_3d":
+        return P.thick_3d(), "3D 32^3 hex, DG Q1, LS-S8 (80 directions), 4^3-cell blocks, c~1"
+    if name == "batch_3d":
+        return P.batch_3d(n=64), "3D 64^3 hex, DG Q1, LS-S12 (168 directions), 8^3-cell blocks"
+    raise SystemExit(f"unknown workload {name}")
+
+
+def cpu_baseline(spec, seconds_budget, threads):
+    """Times the FP64 oracle sweep (plain C restatement of
+    RadiativeTransfer::sweep, one cell per task, Crout LU per cell) on a slab
+    of the same mesh: the first z-layers (all of x, y) with the same
+    materials, one direction per thread.  Returns updates/s over all threads."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle import oracle as O
+    O.lib()
+    n = list(spec.n)
+    # slab size chosen so one direction takes roughly seconds_budget/2
+    per_cell_s = 10e-6 * (spec.n_loc / 27.0) ** 2.2
+    layers = max(1, min(n[-1], int(seconds_budget / 2 / per_cell_s / (n[0] * n[1]))))
+    sub_n = n[:-1] + [layers] if spec.dim == 3 else n[:-1] + [layers]
+    nc = int(np.prod(sub_n))
+    q = spec.quad
+    pb = O.Problem(spec.dim, spec.p, sub_n, spec.h, q.omega, q.M2D, q.D2M, q.mom_l,
+                   spec.sigma_t, spec.sigma_s, spec.mat[:nc], q=spec.q[:nc], ws=spec.ws)
+    S = np.random.default_rng(0).random(pb.size)
+    dirs = [int(d) for d in np.linspace(0, q.n_dir - 1, threads)]
+    out = [None] * threads
+
+    def run(k):
+        out[k] = pb.sweep(S, d_begin=dirs[k], d_end=dirs[k] + 1)
+
+    ths = [threading.Thread(target=run, args=(k,)) for k in range(threads)]
+    t0 = time.perf_counter()
+    for t in ths:
+        t.start()
+    for t in ths:
+        t.join()
+    dt = time.perf_counter() - t0
+    updates = nc * spec.n_loc * threads
+    return {"value": updates / dt, "unit": "DoF-angle updates/s", "cores": threads,
+            "kind": "port",
+            "sample": (f"oracle FP64 sweep of {threads} direction(s) (one per thread) over a "
+                       f"{'x'.join(map(str, sub_n))} slab of the same mesh and materials: "
+                       f"{updates:.3e} updates in {dt:.2f} s")}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="headline_3d")
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path "
+              f"(asked for --gpus {a.gpus}); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import ouranos_amd as ou
+    from ouranos_amd import problems as P
+
+    spec, desc = workload(a.workload)
+    prec = ou.FP32 if a.precision == "fp32" else ou.FP64
+    t0 = time.perf_counter()
+    op = spec.operator(prec)
+    t_plan = time.perf_counter() - t0
+    x = P.seeded_vector(spec.size, 1)
+    d_x = ou.DeviceBuffer(x.nbytes)
+    d_y = ou.DeviceBuffer(x.nbytes)
+    d_x.upload(x)
+    del x
+    # warmup + timed region, device synchronised on both sides
+    op.time_apply(d_x, d_y, a.warmup, 1) if a.warmup > 0 else None
+    ou.lib().ou_device_synchronize()
+    t0 = time.perf_counter()
+    tm = op.time_apply(d_x, d_y, 0, a.steps)
+    ou.lib().ou_device_synchronize()
+    wall = time.perf_counter() - t0
+    y = np.empty(spec.size)
+    d_y.download(y)
+    if not np.isfinite(y).all():
+        raise SystemExit("non-finite output")
+
+    updates = spec.updates_per_sweep()
+    ms_step = 1000.0 * wall / a.steps
+    value = updates * a.steps / wall
+    sweep_ms = tm["ms_sweep_kernel"] / a.steps
+    alg_bytes = spec.algorithmic_bytes_per_sweep()
+    achieved = alg_bytes / (sweep_ms / 1000.0) / 1e9
+    line = {
+        "metric": METRIC, "value": value, "unit": "DoF-angle updates/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": ms_step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32" if prec == ou.FP32 else "fp64", "data": "synthetic",
+        "config": {"workload": f"{spec.name}: {desc}", "n_cells": list(spec.n),
+                   "fe_order": spec.p, "n_dir": spec.quad.n_dir, "n_mom": spec.quad.n_mom,
+                   "dofs": spec.n_dofs, "updates_per_sweep": updates,
+                   "accumulation": "fp64 moments", "seed": spec.meta.get("seed")},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_sweep_level (all level launches of one sweep)",
+                     "algorithmic_bytes_per_sweep": alg_bytes,
+                     "sweep_kernel_ms_per_step": sweep_ms,
+                     "launches_per_step": tm["n_sweep_launches"]},
+        "event_ms_per_step": tm["ms_total"] / a.steps,
+        "plan_seconds": t_plan,
+    }
+    if not a.no_cpu:
+        threads = a.cpu_threads or min(16, os.cpu_count() or 1)
+        line["cpu_baseline"] = cpu_baseline(spec, a.cpu_seconds, threads)
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

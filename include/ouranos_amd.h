@@ -194,6 +194,7 @@ int ou_device_alloc(size_t bytes, void **ptr);
 int ou_device_free(void *ptr);
 int ou_memcpy_htod(void *dst, const void *src, size_t bytes);
 int ou_memcpy_dtoh(void *dst, const void *src, size_t bytes);
+int ou_memcpy_dtod(void *dst, const void *src, size_t bytes);
 int ou_device_synchronize(void);
 int ou_device_name(char *buf, int len);
 

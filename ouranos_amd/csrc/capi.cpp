@@ -351,6 +351,13 @@ int ou_memcpy_dtoh(void *dst, const void *src, size_t bytes) {
   });
 }
 
+int ou_memcpy_dtod(void *dst, const void *src, size_t bytes) {
+  return guarded([&] {
+    OU_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+    return int(OU_OK);
+  });
+}
+
 int ou_device_synchronize(void) {
   return guarded([&] {
     OU_HIP(hipDeviceSynchronize());

@@ -12,6 +12,7 @@
 // cells of one octant and one material.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 
@@ -63,7 +64,7 @@ bool finite_all(const double *p, size_t n) {
 }  // namespace
 
 Plan::~Plan() {
-  void *bufs[] = {d_ainv,   d_S,        d_S0,        d_m2d,     d_omega_abs, d_face_mass,
+  void *bufs[] = {d_ainv_mfma, d_ainv,   d_S,        d_S0,        d_m2d,     d_omega_abs, d_face_mass,
                   d_d2m,    d_partial,  d_trace[0],  d_trace[1], d_trace[2], d_dir_oct,
                   d_oct_dirs, d_oct_off, d_chunks,  d_cells,   d_mat,       d_mom_l,
                   d_sigma_s, d_source,  d_mass,      d_rowsum,  d_inflow,    d_x,
@@ -156,7 +157,16 @@ Plan *create_plan(const ou_problem_desc *desc) {
     if (p.M2D[0] == 0.0) fail(OU_ERR_ARG, "moment 0 of M2D is zero");
 
     p.mm = p.n_mom <= 1 ? 1 : p.n_mom <= 4 ? 4 : p.n_mom <= 10 ? 10 : 16;
-    if (!sweep_supported(p.dim, p.p, p.mm, p.precision))
+    {
+      int dev = 0;
+      OU_HIP(hipGetDevice(&dev));
+      OU_HIP(hipDeviceGetAttribute(&p.max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    }
+    p.use_mfma = p.precision == OU_FP32 && p.N == 27;
+    p.waves = p.use_mfma ? 8 : 4;
+    if (const char *env = std::getenv("OURANOS_SWEEP_WAVES")) p.waves = std::atoi(env);
+    if (!sweep_supported(p.dim, p.p, p.mm, p.precision, p.waves, p.max_lds)) p.waves = 4;
+    if (!sweep_supported(p.dim, p.p, p.mm, p.precision, p.waves, p.max_lds))
       fail(OU_ERR_UNSUPPORTED,
            "no GPU sweep kernel for this (dim, fe_order, n_mom): supported are "
            "2D Q1-Q3 and 3D Q1-Q2 with up to 16 moments");
@@ -255,6 +265,8 @@ Plan *create_plan(const ou_problem_desc *desc) {
     for (auto &e : p.ev) OU_HIP(hipEventCreate(&e));
     const size_t rs = p.real_size();
     p.d_ainv = dev_alloc(size_t(p.n_mat) * p.n_dir * p.N * p.N * rs);
+    if (p.use_mfma)
+      p.d_ainv_mfma = static_cast<float *>(dev_alloc(size_t(p.n_mat) * p.n_dir * 64 * 16 * 4));
     p.d_S = dev_alloc(size_t(p.vec_size) * rs);
     OU_HIP(hipMemset(p.d_S, 0, size_t(p.vec_size) * rs));
     p.d_S0 = dev_alloc(size_t(p.n_dofs) * rs);
@@ -267,7 +279,7 @@ Plan *create_plan(const ou_problem_desc *desc) {
     OU_HIP(hipMemset(p.d_partial, 0, size_t(p.n_oct) * p.vec_size * 8));
     p.plane_size[0] = long(ny) * nz;
     p.plane_size[1] = long(nx) * nz;
-    p.plane_size[2] = long(nx) * ny;
+    p.plane_size[2] = long(nx + ny - 1) * ny;
     for (int a = 0; a < p.dim; ++a)
       p.d_trace[a] = dev_alloc(size_t(p.n_dir) * p.F * p.plane_size[a] * rs);
     p.d_dir_oct = static_cast<int *>(dev_alloc(p.dir_oct.size() * 4));

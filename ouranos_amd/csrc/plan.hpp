@@ -25,6 +25,7 @@ struct SweepView {
   int n_dir, n_mom;
   long n_dofs;
   const void *ainv;        // [mat][dir][N][N] in the sweep precision
+  const float *ainv_mfma;  // [mat][dir][64 lanes][16]: MFMA A-operand layout (FP32, N=27)
   const void *S;           // [m][dof] scattering source (sweep precision)
   const void *S0;          // moment-0 override for the source sweep, or null
   const void *m2d;         // [dir][n_mom] (sweep precision)
@@ -34,7 +35,11 @@ struct SweepView {
   const int *oct_dirs;     // directions grouped by octant
   const int *oct_off;      // [n_oct+1]
   const void *face_mass;   // [axis][F][F] (sweep precision)
-  void *trace[3];          // per axis: [dir][F][plane]
+  // Per axis: [dir][F][slot].  Slot of a cell in the octant's canonical
+  // coordinates (ii,jj,kk): x -> jj + ny kk, y -> ii + nx kk, z -> (ii+jj) ny + jj
+  // (skewed so that the cells of one level with equal kk and consecutive jj
+  // use consecutive slots on every axis).
+  void *trace[3];
   long plane_size[3];
   double *partial;         // [oct][m][dof]
   const Chunk *chunks;
@@ -66,6 +71,9 @@ struct Plan {
   hipStream_t stream = nullptr;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   void *d_ainv = nullptr, *d_S = nullptr, *d_S0 = nullptr, *d_m2d = nullptr;
+  float *d_ainv_mfma = nullptr;
+  bool use_mfma = false;
+  int max_lds = 65536;
   void *d_omega_abs = nullptr, *d_face_mass = nullptr;
   double *d_d2m = nullptr, *d_partial = nullptr;
   void *d_trace[3] = {nullptr, nullptr, nullptr};
@@ -85,7 +93,7 @@ struct Plan {
 };
 
 Plan *create_plan(const ou_problem_desc *desc);
-bool sweep_supported(int dim, int p, int mm, int precision);
+bool sweep_supported(int dim, int p, int mm, int precision, int waves, int max_lds);
 
 // Launch sequences (stream-ordered, no host synchronisation).
 void launch_build_inverses(Plan &pl);

@@ -14,6 +14,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 
 #include "fe.hpp"
 #include "plan.hpp"
@@ -222,18 +223,10 @@ __device__ constexpr int face_dof(int fi) {
 
 // b[upwind face dofs of axis A] += |Omega_A| Mf_A t: the upwind term
 // U_f (|n.Omega| psi_up) of RadiativeTransfer.cc:935-956 restricted to the
-// non-zero block of U_f.
+// non-zero block of U_f.  t holds the upwind trace (prefetched).
 template <typename T, int DIM, int P, int A>
-__device__ __forceinline__ void upwind_axis(const void *trace, long ps, cptr<T> face_mass,
-                                            cptr<T> omega_abs, int d, long plane,
-                                            bool active, T *b) {
+__device__ __forceinline__ void add_upwind(cptr<T> face_mass, T oa, const T *t, T *b) {
   constexpr int P1 = P + 1, F = DIM == 2 ? P1 : P1 * P1;
-  const T oa = omega_abs[d * 3 + A];
-  if (oa == T(0)) return;
-  const T *tr = static_cast<const T *>(trace) + long(d) * F * ps;
-  T t[F];
-#pragma unroll
-  for (int f = 0; f < F; ++f) t[f] = active ? tr[f * ps + plane] : T(0);
   cptr<T> Mf = face_mass + A * F * F;
 #pragma unroll
   for (int fi = 0; fi < F; ++fi) {
@@ -245,12 +238,21 @@ __device__ __forceinline__ void upwind_axis(const void *trace, long ps, cptr<T> 
 }
 
 template <typename T, int DIM, int P, int A>
-__device__ __forceinline__ void downwind_axis(void *trace, long ps, int d, long plane,
-                                              const T *psi) {
+__device__ __forceinline__ void load_trace(const void *trace, long ps, int d, long slot,
+                                           bool active, T *t) {
   constexpr int P1 = P + 1, F = DIM == 2 ? P1 : P1 * P1;
-  T *tr = static_cast<T *>(trace) + long(d) * F * ps;
+  const T *tr = static_cast<const T *>(trace) + long(d) * F * ps + slot;
 #pragma unroll
-  for (int f = 0; f < F; ++f) tr[f * ps + plane] = psi[face_dof<DIM, P, A, P>(f)];
+  for (int f = 0; f < F; ++f) t[f] = active ? tr[f * ps] : T(0);
+}
+
+template <typename T, int DIM, int P, int A>
+__device__ __forceinline__ void store_trace(void *trace, long ps, int d, long slot,
+                                            const T *psi) {
+  constexpr int P1 = P + 1, F = DIM == 2 ? P1 : P1 * P1;
+  T *tr = static_cast<T *>(trace) + long(d) * F * ps + slot;
+#pragma unroll
+  for (int f = 0; f < F; ++f) tr[f * ps] = psi[face_dof<DIM, P, A, P>(f)];
 }
 
 template <typename T, int DIM, int P, int MM, int W>
@@ -260,24 +262,74 @@ struct SweepCfg {
   static constexpr int F = DIM == 2 ? P1 : P1 * P1;
   static constexpr int NP = N | 1;            // odd LDS stride of one cell
   static constexpr int NC = (N + W - 1) / W;  // moment columns per lane
+  // FP32 Q2 hexahedra solve on the matrix cores (see solve_mfma27).
+  static constexpr bool mfma = std::is_same<T, float>::value && N == 27;
   static constexpr size_t psi_bytes = size_t(W) * 64 * NP * sizeof(T);
   static constexpr size_t src_bytes = size_t(MM) * 64 * NP * sizeof(T);
-  static constexpr bool stage_src = psi_bytes + src_bytes <= 65536;
+  static constexpr bool stage_src = psi_bytes + src_bytes <= 98304;
   static constexpr size_t lds_bytes = psi_bytes + (stage_src ? src_bytes : 0);
 };
+
+typedef float v16f __attribute__((ext_vector_type(16)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+// A-operand fragment of the 27x27 inverse for v_mfma_f32_32x32x2_f32:
+// element kk of lane l is Ainv[l % 32][2 kk + l / 32] (zero padded to 32x32).
+struct AFrag {
+  v4f q[4];
+};
+
+__device__ __forceinline__ void load_afrag(const float *base, int lane, AFrag &a) {
+  const v4f *p = reinterpret_cast<const v4f *>(base + lane * 16);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) a.q[k] = p[k];
+}
+
+// psi = Ainv b for 64 cells (lane = cell) on the matrix cores.  The 27 x 64
+// right-hand sides are split into two 32-column blocks (cells 0-31 and
+// 32-63); v_permlane32_swap regroups two rows of b into the B operands of
+// both blocks, 14 k-steps of v_mfma_f32_32x32x2_f32 per block accumulate
+// 32x32 results, and 16 more swaps return every cell's 27 values to its
+// own lane.  Output row i of a block sits in register 4 (i / 8) + i % 4 of
+// lane (i / 4) % 2 * 32 + column.
+__device__ __forceinline__ void solve_mfma27(const AFrag &a, const float *b, float *psi) {
+  v16f accA = {}, accB = {};
+#pragma unroll
+  for (int kk = 0; kk < 14; ++kk) {
+    const float b0 = b[2 * kk];
+    const float b1 = 2 * kk + 1 < 27 ? b[2 * kk + 1] : 0.0f;
+    const auto sw = __builtin_amdgcn_permlane32_swap(__float_as_uint(b0), __float_as_uint(b1),
+                                                     false, false);
+    const float ak = a.q[kk >> 2][kk & 3];
+    accA = __builtin_amdgcn_mfma_f32_32x32x2f32(ak, __uint_as_float(sw[0]), accA, 0, 0, 0);
+    accB = __builtin_amdgcn_mfma_f32_32x32x2f32(ak, __uint_as_float(sw[1]), accB, 0, 0, 0);
+  }
+  float lo[16], hi[16];
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const auto sw = __builtin_amdgcn_permlane32_swap(__float_as_uint(accA[r]),
+                                                     __float_as_uint(accB[r]), false, false);
+    lo[r] = __uint_as_float(sw[0]);
+    hi[r] = __uint_as_float(sw[1]);
+  }
+#pragma unroll
+  for (int i = 0; i < 27; ++i) psi[i] = ((i >> 2) & 1) ? hi[(i >> 3) * 4 + (i & 3)] : lo[(i >> 3) * 4 + (i & 3)];
+}
 
 // Workgroup = one chunk (<= 64 cells of one level, one octant, one material)
 // x all directions of the octant; wave w takes directions w, w+W, ...  Lane
 // = cell.  Per direction a lane builds b from the scattering source and the
-// upwind traces, multiplies by the (material, direction) inverse held in
-// scalar registers, and writes its downwind traces.  The moment accumulation
+// upwind traces, multiplies by the (material, direction) inverse (matrix
+// cores for FP32 Q2 hexahedra, scalar-operand FMAs otherwise) and writes its
+// downwind traces; the traces and inverse of the wave's next direction are
+// loaded while the current one is computed.  The moment accumulation
 // phi_m += D2M[m,d] psi_d (RadiativeTransfer.cc:990-996) is FP64: psi is
 // exchanged through LDS and, after each round of W directions, every wave
 // accumulates a 1/W share of the chunk's (cell, dof) columns.
 template <typename T, int DIM, int P, int MM, int W>
 __global__ __launch_bounds__(64 * W) void k_sweep_level(SweepView v, int chunk_begin) {
   using C = SweepCfg<T, DIM, P, MM, W>;
-  constexpr int N = C::N, NP = C::NP, NC = C::NC;
+  constexpr int N = C::N, NP = C::NP, NC = C::NC, F = C::F;
   extern __shared__ unsigned char lds_raw[];
   T *s_psi = reinterpret_cast<T *>(lds_raw);  // [W][64*NP]
   T *s_src = s_psi + W * 64 * NP;             // [MM][64*NP]
@@ -314,8 +366,8 @@ __global__ __launch_bounds__(64 * W) void k_sweep_level(SweepView v, int chunk_b
   const int ii = (o & 1) ? nx - 1 - ci : ci;
   const int jj = (o & 2) ? ny - 1 - cj : cj;
   const int kk = (DIM == 3 && (o & 4)) ? v.n[2] - 1 - ck : ck;
-  const long plane0 = jj + long(ny) * kk, plane1 = ii + long(nx) * kk,
-             plane2 = ii + long(nx) * jj;
+  const long slot0 = jj + long(ny) * kk, slot1 = ii + long(nx) * kk,
+             slot2 = long(ii + jj) * ny + jj;
 
   double acc[MM][NC];
 #pragma unroll
@@ -332,11 +384,29 @@ __global__ __launch_bounds__(64 * W) void k_sweep_level(SweepView v, int chunk_b
   const int mat = ch.material;
   const T *src_cell = S + long(c) * N;
   const T *src0_cell = (S0 ? S0 : S) + long(c) * N;
+  const float *afrag_base = v.ainv_mfma + long(mat) * n_dir * 64 * 16;
+
+  // Prefetched state of the wave's current direction.
+  T tx[F], ty[F], tz[F];
+  AFrag af;
+  auto prefetch = [&](int slot, T *px, T *py, T *pz, AFrag &pa) {
+    const int d = dirs[slot];
+    load_trace<T, DIM, P, 0>(v.trace[0], v.plane_size[0], d, slot0, active, px);
+    load_trace<T, DIM, P, 1>(v.trace[1], v.plane_size[1], d, slot1, active, py);
+    if constexpr (DIM == 3) load_trace<T, DIM, P, 2>(v.trace[2], v.plane_size[2], d, slot2, active, pz);
+    if constexpr (C::mfma) load_afrag(afrag_base + long(d) * 64 * 16, lane, pa);
+  };
+  if (w < ndir_o) prefetch(w, tx, ty, tz, af);
 
   for (int r0 = 0; r0 < ndir_o; r0 += W) {
     const int slot = r0 + w;
     if (slot < ndir_o) {
       const int d = dirs[slot];
+      T nx_[F], ny_[F], nz_[F];
+      AFrag naf;
+      const bool more = slot + W < ndir_o;
+      if (more) prefetch(slot + W, nx_, ny_, nz_, naf);
+
       T b[N];
 #pragma unroll
       for (int i = 0; i < N; ++i) b[i] = T(0);
@@ -354,32 +424,46 @@ __global__ __launch_bounds__(64 * W) void k_sweep_level(SweepView v, int chunk_b
             b[i] += cm * (active ? src[natural_index<DIM, P>(i, o)] : T(0));
         }
       }
-      upwind_axis<T, DIM, P, 0>(v.trace[0], v.plane_size[0], face_mass, omega_abs, d, plane0,
-                                active, b);
-      upwind_axis<T, DIM, P, 1>(v.trace[1], v.plane_size[1], face_mass, omega_abs, d, plane1,
-                                active, b);
-      if constexpr (DIM == 3)
-        upwind_axis<T, DIM, P, 2>(v.trace[2], v.plane_size[2], face_mass, omega_abs, d,
-                                  plane2, active, b);
+      {
+        const T o0 = omega_abs[d * 3 + 0], o1 = omega_abs[d * 3 + 1];
+        if (o0 != T(0)) add_upwind<T, DIM, P, 0>(face_mass, o0, tx, b);
+        if (o1 != T(0)) add_upwind<T, DIM, P, 1>(face_mass, o1, ty, b);
+        if constexpr (DIM == 3) {
+          const T o2 = omega_abs[d * 3 + 2];
+          if (o2 != T(0)) add_upwind<T, DIM, P, 2>(face_mass, o2, tz, b);
+        }
+      }
 
-      cptr<T> Ai = as_const<T>(v.ainv) + (long(mat) * n_dir + d) * (N * N);
       T psi[N];
+      if constexpr (C::mfma) {
+        solve_mfma27(af, reinterpret_cast<const float *>(b), reinterpret_cast<float *>(psi));
+      } else {
+        cptr<T> Ai = as_const<T>(v.ainv) + (long(mat) * n_dir + d) * (N * N);
 #pragma unroll
-      for (int r = 0; r < N; ++r) {
-        T s = T(0);
+        for (int r = 0; r < N; ++r) {
+          T s = T(0);
 #pragma unroll
-        for (int q = 0; q < N; ++q) s += Ai[r * N + q] * b[q];
-        psi[r] = s;
+          for (int q = 0; q < N; ++q) s += Ai[r * N + q] * b[q];
+          psi[r] = s;
+        }
       }
       if (active) {
-        downwind_axis<T, DIM, P, 0>(v.trace[0], v.plane_size[0], d, plane0, psi);
-        downwind_axis<T, DIM, P, 1>(v.trace[1], v.plane_size[1], d, plane1, psi);
-        if constexpr (DIM == 3)
-          downwind_axis<T, DIM, P, 2>(v.trace[2], v.plane_size[2], d, plane2, psi);
+        store_trace<T, DIM, P, 0>(v.trace[0], v.plane_size[0], d, slot0, psi);
+        store_trace<T, DIM, P, 1>(v.trace[1], v.plane_size[1], d, slot1, psi);
+        if constexpr (DIM == 3) store_trace<T, DIM, P, 2>(v.trace[2], v.plane_size[2], d, slot2, psi);
       }
       T *dst = s_psi + w * 64 * NP + lane * NP;
 #pragma unroll
       for (int i = 0; i < N; ++i) dst[natural_index<DIM, P>(i, o)] = psi[i];
+      if (more) {
+#pragma unroll
+        for (int f = 0; f < F; ++f) {
+          tx[f] = nx_[f];
+          ty[f] = ny_[f];
+          if constexpr (DIM == 3) tz[f] = nz_[f];
+        }
+        if constexpr (C::mfma) af = naf;
+      }
     }
     __syncthreads();
 #pragma unroll
@@ -413,6 +497,21 @@ __global__ __launch_bounds__(64 * W) void k_sweep_level(SweepView v, int chunk_b
           if (m < n_mom) out[long(m) * n_dofs] = acc[m][t];
       }
     }
+  }
+}
+
+// Rearranges the canonical row-major inverses of FP32 Q2 hexahedra into the
+// A-operand layout of solve_mfma27: out[md][lane][kk] = A[lane%32][2kk+lane/32].
+__global__ __launch_bounds__(64) void k_ainv_to_mfma(const float *__restrict__ ainv,
+                                                     float *__restrict__ out) {
+  const int lane = threadIdx.x;
+  const long md = blockIdx.x;
+  const float *A = ainv + md * 27 * 27;
+  float *o = out + (md * 64 + lane) * 16;
+  const int m = lane & 31;
+  for (int kk = 0; kk < 16; ++kk) {
+    const int k = 2 * kk + (lane >> 5);
+    o[kk] = (m < 27 && k < 27) ? A[m * 27 + k] : 0.0f;
   }
 }
 
@@ -462,20 +561,32 @@ LevelLauncher pick_mm(int mm) {
   switch (mm) {
     case 1: return launch_levels_impl<T, DIM, P, 1, W>;
     case 4: return launch_levels_impl<T, DIM, P, 4, W>;
-    case 10: return launch_levels_impl<T, DIM, P, 10, W>;
-    case 16: return launch_levels_impl<T, DIM, P, 16, W>;
+    case 10: if constexpr (W == 4) return launch_levels_impl<T, DIM, P, 10, W>; break;
+    case 16: if constexpr (W == 4) return launch_levels_impl<T, DIM, P, 16, W>; break;
   }
   return nullptr;
 }
 
 template <typename T>
-LevelLauncher pick(int dim, int p, int mm) {
-  if (dim == 2 && p == 1) return pick_mm<T, 2, 1, 4>(mm);
-  if (dim == 2 && p == 2) return pick_mm<T, 2, 2, 4>(mm);
-  if (dim == 2 && p == 3) return pick_mm<T, 2, 3, 4>(mm);
-  if (dim == 3 && p == 1) return pick_mm<T, 3, 1, 4>(mm);
-  if (dim == 3 && p == 2) return pick_mm<T, 3, 2, 4>(mm);
+LevelLauncher pick(int dim, int p, int mm, int waves) {
+  if (waves == 4) {
+    if (dim == 2 && p == 1) return pick_mm<T, 2, 1, 4>(mm);
+    if (dim == 2 && p == 2) return pick_mm<T, 2, 2, 4>(mm);
+    if (dim == 2 && p == 3) return pick_mm<T, 2, 3, 4>(mm);
+    if (dim == 3 && p == 1) return pick_mm<T, 3, 1, 4>(mm);
+    if (dim == 3 && p == 2) return pick_mm<T, 3, 2, 4>(mm);
+  } else if (waves == 8) {
+    if (dim == 3 && p == 2 && mm <= 4) return pick_mm<T, 3, 2, 8>(mm);
+  }
   return nullptr;
+}
+
+template <typename T>
+size_t lds_need(int dim, int p, int mm, int waves) {
+  const int P1 = p + 1, N = dim == 2 ? P1 * P1 : P1 * P1 * P1, NP = N | 1;
+  const size_t psi = size_t(waves) * 64 * NP * sizeof(T);
+  const size_t src = size_t(mm) * 64 * NP * sizeof(T);
+  return psi + (psi + src <= 98304 ? src : 0) + 256;
 }
 
 template <typename T>
@@ -539,6 +650,7 @@ SweepView Plan::view() const {
   v.n_mom = n_mom;
   v.n_dofs = n_dofs;
   v.ainv = d_ainv;
+  v.ainv_mfma = d_ainv_mfma;
   v.S = d_S;
   v.S0 = nullptr;
   v.m2d = d_m2d;
@@ -591,6 +703,10 @@ void launch_build_inverses(Plan &pl) {
                        pl.dim, pl.p, pl.n_dir, dom, dst, dM, dG, dE,
                        static_cast<double *>(pl.d_ainv));
   OU_HIP(hipGetLastError());
+  if (pl.use_mfma)
+    hipLaunchKernelGGL(k_ainv_to_mfma, dim3(blocks), dim3(64), 0, pl.stream,
+                       static_cast<const float *>(pl.d_ainv), pl.d_ainv_mfma);
+  OU_HIP(hipGetLastError());
   OU_HIP(hipStreamSynchronize(pl.stream));
   hipFree(dM);
   hipFree(dG);
@@ -599,9 +715,12 @@ void launch_build_inverses(Plan &pl) {
   hipFree(dst);
 }
 
-bool sweep_supported(int dim, int p, int mm, int precision) {
-  return precision == OU_FP32 ? pick<float>(dim, p, mm) != nullptr
-                              : pick<double>(dim, p, mm) != nullptr;
+bool sweep_supported(int dim, int p, int mm, int precision, int waves, int max_lds) {
+  if (precision == OU_FP32)
+    return pick<float>(dim, p, mm, waves) != nullptr &&
+           lds_need<float>(dim, p, mm, waves) <= size_t(max_lds);
+  return pick<double>(dim, p, mm, waves) != nullptr &&
+         lds_need<double>(dim, p, mm, waves) <= size_t(max_lds);
 }
 
 void launch_scatter_source(Plan &pl, const double *d_x, hipStream_t s) {
@@ -630,8 +749,8 @@ void launch_sweep_levels(Plan &pl, const void *S0, hipStream_t s, hipEvent_t bef
                          hipEvent_t after) {
   SweepView v = pl.view();
   v.S0 = S0;
-  LevelLauncher f = pl.precision == OU_FP32 ? pick<float>(pl.dim, pl.p, pl.mm)
-                                            : pick<double>(pl.dim, pl.p, pl.mm);
+  LevelLauncher f = pl.precision == OU_FP32 ? pick<float>(pl.dim, pl.p, pl.mm, pl.waves)
+                                            : pick<double>(pl.dim, pl.p, pl.mm, pl.waves);
   if (!f) fail(OU_ERR_UNSUPPORTED, "no sweep kernel for this element / moment count");
   f(pl, v, s, before, after);
 }
