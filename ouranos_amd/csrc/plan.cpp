@@ -11,6 +11,7 @@
 // the octant.  The schedule below lists, per level, chunks of at most 64
 // cells of one octant and one material.
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -64,11 +65,11 @@ bool finite_all(const double *p, size_t n) {
 }  // namespace
 
 Plan::~Plan() {
-  void *bufs[] = {d_ainv_mfma, d_ainv,   d_S,        d_S0,        d_m2d,     d_omega_abs, d_face_mass,
-                  d_d2m,    d_partial,  d_trace[0],  d_trace[1], d_trace[2], d_dir_oct,
-                  d_oct_dirs, d_oct_off, d_chunks,  d_cells,   d_mat,       d_mom_l,
-                  d_sigma_s, d_source,  d_mass,      d_rowsum,  d_inflow,    d_x,
-                  d_y};
+  void *bufs[] = {d_ainv_mfma, d_ainv,     d_S,        d_S0,       d_slot_m2d, d_slot_c,
+                  d_slot_d2m,  d_partial,  d_trace[0], d_trace[1], d_trace[2], d_oct_dirs,
+                  d_oct_off,   d_chunks,   d_cells,    d_mat,      d_mom_l,    d_sigma_s,
+                  d_source,    d_mass,     d_rowsum,   d_inflow,   d_x,        d_y,
+                  d_slot_cls};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   for (hipEvent_t e : ev)
@@ -191,6 +192,32 @@ Plan *create_plan(const ou_problem_desc *desc) {
       std::vector<int> fill(p.oct_off.begin(), p.oct_off.end() - 1);
       for (int d = 0; d < p.n_dir; ++d) p.oct_dirs[fill[p.dir_oct[d]]++] = d;
     }
+    // Inverse classes: in the canonical frame of its octant the cell matrix of
+    // a direction depends on |Omega_a| (a < dim) only, so directions that
+    // mirror into each other share one inverse per material.
+    {
+      std::vector<std::array<double, 3>> keys;
+      p.dir_cls.assign(p.n_dir, -1);
+      for (int d = 0; d < p.n_dir; ++d) {
+        std::array<double, 3> k = {0.0, 0.0, 0.0};
+        for (int a = 0; a < p.dim; ++a) k[a] = omega_abs[3 * d + a];
+        int found = -1;
+        for (size_t c = 0; c < keys.size(); ++c)
+          if (keys[c] == k) {
+            found = int(c);
+            break;
+          }
+        if (found < 0) {
+          found = int(keys.size());
+          keys.push_back(k);
+        }
+        p.dir_cls[d] = found;
+      }
+      p.n_cls = int(keys.size());
+      p.cls_omega.resize(3 * size_t(p.n_cls));
+      for (int c = 0; c < p.n_cls; ++c)
+        for (int a = 0; a < 3; ++a) p.cls_omega[3 * c + a] = keys[c][a];
+    }
 
     // ---- wavefront schedule ------------------------------------------------
     const int nx = p.n[0], ny = p.n[1], nz = p.n[2];
@@ -250,31 +277,50 @@ Plan *create_plan(const ou_problem_desc *desc) {
 
     // ---- element data -------------------------------------------------------
     RefElement re(p.dim, p.p, p.h);
-    std::vector<double> mass, fm, face_mass(3 * size_t(p.F) * p.F, 0.0);
+    std::vector<double> mass;
     re.mass(mass);
     std::vector<double> rowsum(p.N, 0.0);
     for (int i = 0; i < p.N; ++i)
       for (int j = 0; j < p.N; ++j) rowsum[i] += mass[size_t(i) * p.N + j];
-    for (int a = 0; a < p.dim; ++a) {
-      re.face_mass(a, fm);
-      std::copy(fm.begin(), fm.end(), face_mass.begin() + size_t(a) * p.F * p.F);
+    for (int k = 0; k < p.P1 * p.P1; ++k) p.m1[k] = re.m1[k];
+
+    // ---- per-slot direction tables (slot g = direction oct_dirs[g]) ------------
+    // The face mass of axis a is (area_a) m1 (x) m1 in 3D and h_b m1 in 2D
+    // (area_a = product of the cell sizes along the other axes); the kernel
+    // applies the m1 factors and takes |Omega_a| area_a from this table.
+    for (int o = 0; o < p.n_oct; ++o)
+      if (p.oct_off[o + 1] - p.oct_off[o] > 64)
+        fail(OU_ERR_UNSUPPORTED, "more than 64 directions in one octant");
+    std::vector<double> slot_m2d(size_t(p.n_dir) * p.mm, 0.0), slot_d2m(size_t(p.n_dir) * p.mm, 0.0),
+        slot_c(size_t(p.n_dir) * 4, 0.0);
+    for (int g = 0; g < p.n_dir; ++g) {
+      const int d = p.oct_dirs[g];
+      for (int m = 0; m < p.n_mom; ++m) {
+        slot_m2d[size_t(g) * p.mm + m] = p.M2D[size_t(d) * p.n_mom + m];
+        slot_d2m[size_t(g) * p.mm + m] = p.D2M[size_t(m) * p.n_dir + d];
+      }
+      for (int a = 0; a < p.dim; ++a) {
+        double area = 1.0;
+        for (int b2 = 0; b2 < p.dim; ++b2)
+          if (b2 != a) area *= p.h[b2];
+        slot_c[size_t(g) * 4 + a] = omega_abs[3 * d + a] * area;
+      }
     }
 
     // ---- device buffers -----------------------------------------------------
     OU_HIP(hipStreamCreateWithFlags(&p.stream, hipStreamNonBlocking));
     for (auto &e : p.ev) OU_HIP(hipEventCreate(&e));
     const size_t rs = p.real_size();
-    p.d_ainv = dev_alloc(size_t(p.n_mat) * p.n_dir * p.N * p.N * rs);
+    p.d_ainv = dev_alloc(size_t(p.n_mat) * p.n_cls * p.N * p.N * rs);
     if (p.use_mfma)
-      p.d_ainv_mfma = static_cast<float *>(dev_alloc(size_t(p.n_mat) * p.n_dir * 64 * 16 * 4));
+      p.d_ainv_mfma = static_cast<float *>(dev_alloc(size_t(p.n_mat) * p.n_cls * 64 * 16 * 4));
     p.d_S = dev_alloc(size_t(p.vec_size) * rs);
     OU_HIP(hipMemset(p.d_S, 0, size_t(p.vec_size) * rs));
     p.d_S0 = dev_alloc(size_t(p.n_dofs) * rs);
-    p.d_m2d = upload_real(p.M2D, p.precision);
-    p.d_omega_abs = upload_real(omega_abs, p.precision);
-    p.d_face_mass = upload_real(face_mass, p.precision);
-    p.d_d2m = static_cast<double *>(dev_alloc(p.D2M.size() * 8));
-    upload(p.d_d2m, p.D2M);
+    p.d_slot_m2d = upload_real(slot_m2d, p.precision);
+    p.d_slot_c = upload_real(slot_c, p.precision);
+    p.d_slot_d2m = static_cast<double *>(dev_alloc(slot_d2m.size() * 8));
+    upload(p.d_slot_d2m, slot_d2m);
     p.d_partial = static_cast<double *>(dev_alloc(size_t(p.n_oct) * p.vec_size * 8));
     OU_HIP(hipMemset(p.d_partial, 0, size_t(p.n_oct) * p.vec_size * 8));
     p.plane_size[0] = long(ny) * nz;
@@ -282,10 +328,14 @@ Plan *create_plan(const ou_problem_desc *desc) {
     p.plane_size[2] = long(nx + ny - 1) * ny;
     for (int a = 0; a < p.dim; ++a)
       p.d_trace[a] = dev_alloc(size_t(p.n_dir) * p.F * p.plane_size[a] * rs);
-    p.d_dir_oct = static_cast<int *>(dev_alloc(p.dir_oct.size() * 4));
-    upload(p.d_dir_oct, p.dir_oct);
     p.d_oct_dirs = static_cast<int *>(dev_alloc(p.oct_dirs.size() * 4));
     upload(p.d_oct_dirs, p.oct_dirs);
+    {
+      std::vector<int> slot_cls(p.n_dir);
+      for (int g = 0; g < p.n_dir; ++g) slot_cls[g] = p.dir_cls[p.oct_dirs[g]];
+      p.d_slot_cls = static_cast<int *>(dev_alloc(slot_cls.size() * 4));
+      upload(p.d_slot_cls, slot_cls);
+    }
     p.d_oct_off = static_cast<int *>(dev_alloc(p.oct_off.size() * 4));
     upload(p.d_oct_off, p.oct_off);
     p.d_chunks = static_cast<Chunk *>(dev_alloc(chunks.size() * sizeof(Chunk)));

@@ -24,17 +24,23 @@ struct SweepView {
   int n[3];
   int n_dir, n_mom;
   long n_dofs;
-  const void *ainv;        // [mat][dir][N][N] in the sweep precision
-  const float *ainv_mfma;  // [mat][dir][64 lanes][16]: MFMA A-operand layout (FP32, N=27)
+  // Inverses per (material, class): a class is a set of directions with equal
+  // |Omega_a| on every axis, which share one matrix in their octants'
+  // canonical frames.
+  int n_cls;
+  const int *slot_cls;     // [slot] class of direction oct_dirs[slot]
+  const void *ainv;        // [mat][cls][N][N] in the sweep precision (canonical frame)
+  const float *ainv_mfma;  // [mat][cls][64 lanes][16]: MFMA A-operand layout (FP32, N=27)
   const void *S;           // [m][dof] scattering source (sweep precision)
   const void *S0;          // moment-0 override for the source sweep, or null
-  const void *m2d;         // [dir][n_mom] (sweep precision)
-  const double *d2m;       // [m][dir]
-  const void *omega_abs;   // [dir][3] |Omega_a| (sweep precision)
-  const int *dir_oct;      // [dir] octant id
-  const int *oct_dirs;     // directions grouped by octant
+  const int *oct_dirs;     // directions grouped by octant ("slots")
   const int *oct_off;      // [n_oct+1]
-  const void *face_mass;   // [axis][F][F] (sweep precision)
+  // Per slot g (direction oct_dirs[g]), padded to MM moments:
+  const void *slot_m2d;    // [g][MM] M2D row (sweep precision)
+  const double *slot_d2m;  // [g][MM] D2M column
+  const void *slot_c;      // [g][4] |Omega_a| x (area of the face normal to a) (sweep precision)
+  float m1f[16];           // 1D reference mass matrix on [0,1]
+  double m1d[16];
   // Per axis: [dir][F][slot].  Slot of a cell in the octant's canonical
   // coordinates (ii,jj,kk): x -> jj + ny kk, y -> ii + nx kk, z -> (ii+jj) ny + jj
   // (skewed so that the cells of one level with equal kk and consecutive jj
@@ -64,21 +70,26 @@ struct Plan {
   // schedule
   int n_oct = 4, n_levels = 0, waves = 4, mm = 1;
   std::vector<int> oct_dirs, oct_off, dir_oct;
+  int n_cls = 0;
+  std::vector<int> dir_cls;          // [dir] inverse class
+  std::vector<double> cls_omega;     // [cls][3] |Omega| of the class
+  int *d_slot_cls = nullptr;
   std::vector<int> level_chunk_off;  // [n_levels+1]
   long n_chunks = 0;
 
   // device buffers
   hipStream_t stream = nullptr;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-  void *d_ainv = nullptr, *d_S = nullptr, *d_S0 = nullptr, *d_m2d = nullptr;
+  void *d_ainv = nullptr, *d_S = nullptr, *d_S0 = nullptr;
   float *d_ainv_mfma = nullptr;
   bool use_mfma = false;
   int max_lds = 65536;
-  void *d_omega_abs = nullptr, *d_face_mass = nullptr;
-  double *d_d2m = nullptr, *d_partial = nullptr;
+  void *d_slot_m2d = nullptr, *d_slot_c = nullptr;
+  double *d_slot_d2m = nullptr, *d_partial = nullptr;
+  double m1[16] = {};
   void *d_trace[3] = {nullptr, nullptr, nullptr};
   long plane_size[3] = {0, 0, 0};
-  int *d_dir_oct = nullptr, *d_oct_dirs = nullptr, *d_oct_off = nullptr;
+  int *d_oct_dirs = nullptr, *d_oct_off = nullptr;
   Chunk *d_chunks = nullptr;
   int *d_cells = nullptr, *d_mat = nullptr, *d_mom_l = nullptr;
   double *d_sigma_s = nullptr, *d_source = nullptr, *d_mass = nullptr;

@@ -35,7 +35,9 @@ __device__ __forceinline__ cptr<T> as_const(const void *p) {
 }
 
 // ------------------------------------------------------------------------
-// Inverse matrices: one 64-lane wave per (material, direction).
+// Inverse matrices: one 64-lane wave per (material, direction class).  The
+// plan passes one representative |Omega| per class (all components >= 0),
+// so the matrix is assembled directly in the canonical frame.
 // ------------------------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(64) void k_build_inverses(
@@ -221,284 +223,7 @@ __device__ constexpr int face_dof(int fi) {
                 : (A == 1 ? u + P1 * (SIDE + P1 * v) : u + P1 * (v + P1 * SIDE));
 }
 
-// b[upwind face dofs of axis A] += |Omega_A| Mf_A t: the upwind term
-// U_f (|n.Omega| psi_up) of RadiativeTransfer.cc:935-956 restricted to the
-// non-zero block of U_f.  t holds the upwind trace (prefetched).
-template <typename T, int DIM, int P, int A>
-__device__ __forceinline__ void add_upwind(cptr<T> face_mass, T oa, const T *t, T *b) {
-  constexpr int P1 = P + 1, F = DIM == 2 ? P1 : P1 * P1;
-  cptr<T> Mf = face_mass + A * F * F;
-#pragma unroll
-  for (int fi = 0; fi < F; ++fi) {
-    T s = T(0);
-#pragma unroll
-    for (int fj = 0; fj < F; ++fj) s += Mf[fi * F + fj] * t[fj];
-    b[face_dof<DIM, P, A, 0>(fi)] += oa * s;
-  }
-}
-
-template <typename T, int DIM, int P, int A>
-__device__ __forceinline__ void load_trace(const void *trace, long ps, int d, long slot,
-                                           bool active, T *t) {
-  constexpr int P1 = P + 1, F = DIM == 2 ? P1 : P1 * P1;
-  const T *tr = static_cast<const T *>(trace) + long(d) * F * ps + slot;
-#pragma unroll
-  for (int f = 0; f < F; ++f) t[f] = active ? tr[f * ps] : T(0);
-}
-
-template <typename T, int DIM, int P, int A>
-__device__ __forceinline__ void store_trace(void *trace, long ps, int d, long slot,
-                                            const T *psi) {
-  constexpr int P1 = P + 1, F = DIM == 2 ? P1 : P1 * P1;
-  T *tr = static_cast<T *>(trace) + long(d) * F * ps + slot;
-#pragma unroll
-  for (int f = 0; f < F; ++f) tr[f * ps] = psi[face_dof<DIM, P, A, P>(f)];
-}
-
-template <typename T, int DIM, int P, int MM, int W>
-struct SweepCfg {
-  static constexpr int P1 = P + 1;
-  static constexpr int N = DIM == 2 ? P1 * P1 : P1 * P1 * P1;
-  static constexpr int F = DIM == 2 ? P1 : P1 * P1;
-  static constexpr int NP = N | 1;            // odd LDS stride of one cell
-  static constexpr int NC = (N + W - 1) / W;  // moment columns per lane
-  // FP32 Q2 hexahedra solve on the matrix cores (see solve_mfma27).
-  static constexpr bool mfma = std::is_same<T, float>::value && N == 27;
-  static constexpr size_t psi_bytes = size_t(W) * 64 * NP * sizeof(T);
-  static constexpr size_t src_bytes = size_t(MM) * 64 * NP * sizeof(T);
-  static constexpr bool stage_src = psi_bytes + src_bytes <= 98304;
-  static constexpr size_t lds_bytes = psi_bytes + (stage_src ? src_bytes : 0);
-};
-
-typedef float v16f __attribute__((ext_vector_type(16)));
-typedef float v4f __attribute__((ext_vector_type(4)));
-
-// A-operand fragment of the 27x27 inverse for v_mfma_f32_32x32x2_f32:
-// element kk of lane l is Ainv[l % 32][2 kk + l / 32] (zero padded to 32x32).
-struct AFrag {
-  v4f q[4];
-};
-
-__device__ __forceinline__ void load_afrag(const float *base, int lane, AFrag &a) {
-  const v4f *p = reinterpret_cast<const v4f *>(base + lane * 16);
-#pragma unroll
-  for (int k = 0; k < 4; ++k) a.q[k] = p[k];
-}
-
-// psi = Ainv b for 64 cells (lane = cell) on the matrix cores.  The 27 x 64
-// right-hand sides are split into two 32-column blocks (cells 0-31 and
-// 32-63); v_permlane32_swap regroups two rows of b into the B operands of
-// both blocks, 14 k-steps of v_mfma_f32_32x32x2_f32 per block accumulate
-// 32x32 results, and 16 more swaps return every cell's 27 values to its
-// own lane.  Output row i of a block sits in register 4 (i / 8) + i % 4 of
-// lane (i / 4) % 2 * 32 + column.
-__device__ __forceinline__ void solve_mfma27(const AFrag &a, const float *b, float *psi) {
-  v16f accA = {}, accB = {};
-#pragma unroll
-  for (int kk = 0; kk < 14; ++kk) {
-    const float b0 = b[2 * kk];
-    const float b1 = 2 * kk + 1 < 27 ? b[2 * kk + 1] : 0.0f;
-    const auto sw = __builtin_amdgcn_permlane32_swap(__float_as_uint(b0), __float_as_uint(b1),
-                                                     false, false);
-    const float ak = a.q[kk >> 2][kk & 3];
-    accA = __builtin_amdgcn_mfma_f32_32x32x2f32(ak, __uint_as_float(sw[0]), accA, 0, 0, 0);
-    accB = __builtin_amdgcn_mfma_f32_32x32x2f32(ak, __uint_as_float(sw[1]), accB, 0, 0, 0);
-  }
-  float lo[16], hi[16];
-#pragma unroll
-  for (int r = 0; r < 16; ++r) {
-    const auto sw = __builtin_amdgcn_permlane32_swap(__float_as_uint(accA[r]),
-                                                     __float_as_uint(accB[r]), false, false);
-    lo[r] = __uint_as_float(sw[0]);
-    hi[r] = __uint_as_float(sw[1]);
-  }
-#pragma unroll
-  for (int i = 0; i < 27; ++i) psi[i] = ((i >> 2) & 1) ? hi[(i >> 3) * 4 + (i & 3)] : lo[(i >> 3) * 4 + (i & 3)];
-}
-
-// Workgroup = one chunk (<= 64 cells of one level, one octant, one material)
-// x all directions of the octant; wave w takes directions w, w+W, ...  Lane
-// = cell.  Per direction a lane builds b from the scattering source and the
-// upwind traces, multiplies by the (material, direction) inverse (matrix
-// cores for FP32 Q2 hexahedra, scalar-operand FMAs otherwise) and writes its
-// downwind traces; the traces and inverse of the wave's next direction are
-// loaded while the current one is computed.  The moment accumulation
-// phi_m += D2M[m,d] psi_d (RadiativeTransfer.cc:990-996) is FP64: psi is
-// exchanged through LDS and, after each round of W directions, every wave
-// accumulates a 1/W share of the chunk's (cell, dof) columns.
-template <typename T, int DIM, int P, int MM, int W>
-__global__ __launch_bounds__(64 * W) void k_sweep_level(SweepView v, int chunk_begin) {
-  using C = SweepCfg<T, DIM, P, MM, W>;
-  constexpr int N = C::N, NP = C::NP, NC = C::NC, F = C::F;
-  extern __shared__ unsigned char lds_raw[];
-  T *s_psi = reinterpret_cast<T *>(lds_raw);  // [W][64*NP]
-  T *s_src = s_psi + W * 64 * NP;             // [MM][64*NP]
-  __shared__ int s_cell[64];
-
-  const Chunk ch = v.chunks[chunk_begin + blockIdx.x];
-  const int tid = threadIdx.x, lane = tid & 63;
-  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int cnt = ch.count, o = ch.octant;
-  const bool active = lane < cnt;
-  if (tid < 64) s_cell[tid] = tid < cnt ? v.cells[ch.cell_offset + tid] : 0;
-  __syncthreads();
-
-  const T *S = static_cast<const T *>(v.S);
-  const T *S0 = static_cast<const T *>(v.S0);
-  const long n_dofs = v.n_dofs;
-  const int n_mom = v.n_mom, n_dir = v.n_dir;
-  if constexpr (C::stage_src) {
-    for (int e = tid; e < n_mom * 64 * N; e += 64 * W) {
-      const int m = e / (64 * N), r = e - m * 64 * N, cl = r / N, i = r - cl * N;
-      T val = T(0);
-      if (cl < cnt) {
-        const long g = long(s_cell[cl]) * N + i;
-        val = (m == 0 && S0) ? S0[g] : S[long(m) * n_dofs + g];
-      }
-      s_src[m * 64 * NP + cl * NP + i] = val;
-    }
-    __syncthreads();
-  }
-
-  const int c = s_cell[lane];
-  const int nx = v.n[0], ny = v.n[1];
-  const int ci = c % nx, cj = (c / nx) % ny, ck = DIM == 3 ? c / (nx * ny) : 0;
-  const int ii = (o & 1) ? nx - 1 - ci : ci;
-  const int jj = (o & 2) ? ny - 1 - cj : cj;
-  const int kk = (DIM == 3 && (o & 4)) ? v.n[2] - 1 - ck : ck;
-  const long slot0 = jj + long(ny) * kk, slot1 = ii + long(nx) * kk,
-             slot2 = long(ii + jj) * ny + jj;
-
-  double acc[MM][NC];
-#pragma unroll
-  for (int m = 0; m < MM; ++m)
-#pragma unroll
-    for (int t = 0; t < NC; ++t) acc[m][t] = 0.0;
-
-  const int dbeg = v.oct_off[o], ndir_o = v.oct_off[o + 1] - dbeg;
-  cptr<int> dirs = as_const<int>(v.oct_dirs) + dbeg;
-  cptr<T> m2d = as_const<T>(v.m2d);
-  cptr<double> d2m = as_const<double>(v.d2m);
-  cptr<T> omega_abs = as_const<T>(v.omega_abs);
-  cptr<T> face_mass = as_const<T>(v.face_mass);
-  const int mat = ch.material;
-  const T *src_cell = S + long(c) * N;
-  const T *src0_cell = (S0 ? S0 : S) + long(c) * N;
-  const float *afrag_base = v.ainv_mfma + long(mat) * n_dir * 64 * 16;
-
-  // Prefetched state of the wave's current direction.
-  T tx[F], ty[F], tz[F];
-  AFrag af;
-  auto prefetch = [&](int slot, T *px, T *py, T *pz, AFrag &pa) {
-    const int d = dirs[slot];
-    load_trace<T, DIM, P, 0>(v.trace[0], v.plane_size[0], d, slot0, active, px);
-    load_trace<T, DIM, P, 1>(v.trace[1], v.plane_size[1], d, slot1, active, py);
-    if constexpr (DIM == 3) load_trace<T, DIM, P, 2>(v.trace[2], v.plane_size[2], d, slot2, active, pz);
-    if constexpr (C::mfma) load_afrag(afrag_base + long(d) * 64 * 16, lane, pa);
-  };
-  if (w < ndir_o) prefetch(w, tx, ty, tz, af);
-
-  for (int r0 = 0; r0 < ndir_o; r0 += W) {
-    const int slot = r0 + w;
-    if (slot < ndir_o) {
-      const int d = dirs[slot];
-      T nx_[F], ny_[F], nz_[F];
-      AFrag naf;
-      const bool more = slot + W < ndir_o;
-      if (more) prefetch(slot + W, nx_, ny_, nz_, naf);
-
-      T b[N];
-#pragma unroll
-      for (int i = 0; i < N; ++i) b[i] = T(0);
-      for (int m = 0; m < MM; ++m) {
-        if (m >= n_mom) break;
-        const T cm = m2d[d * n_mom + m];
-        if constexpr (C::stage_src) {
-          const T *src = s_src + m * 64 * NP + lane * NP;
-#pragma unroll
-          for (int i = 0; i < N; ++i) b[i] += cm * src[natural_index<DIM, P>(i, o)];
-        } else {
-          const T *src = m == 0 ? src0_cell : src_cell + long(m) * n_dofs;
-#pragma unroll
-          for (int i = 0; i < N; ++i)
-            b[i] += cm * (active ? src[natural_index<DIM, P>(i, o)] : T(0));
-        }
-      }
-      {
-        const T o0 = omega_abs[d * 3 + 0], o1 = omega_abs[d * 3 + 1];
-        if (o0 != T(0)) add_upwind<T, DIM, P, 0>(face_mass, o0, tx, b);
-        if (o1 != T(0)) add_upwind<T, DIM, P, 1>(face_mass, o1, ty, b);
-        if constexpr (DIM == 3) {
-          const T o2 = omega_abs[d * 3 + 2];
-          if (o2 != T(0)) add_upwind<T, DIM, P, 2>(face_mass, o2, tz, b);
-        }
-      }
-
-      T psi[N];
-      if constexpr (C::mfma) {
-        solve_mfma27(af, reinterpret_cast<const float *>(b), reinterpret_cast<float *>(psi));
-      } else {
-        cptr<T> Ai = as_const<T>(v.ainv) + (long(mat) * n_dir + d) * (N * N);
-#pragma unroll
-        for (int r = 0; r < N; ++r) {
-          T s = T(0);
-#pragma unroll
-          for (int q = 0; q < N; ++q) s += Ai[r * N + q] * b[q];
-          psi[r] = s;
-        }
-      }
-      if (active) {
-        store_trace<T, DIM, P, 0>(v.trace[0], v.plane_size[0], d, slot0, psi);
-        store_trace<T, DIM, P, 1>(v.trace[1], v.plane_size[1], d, slot1, psi);
-        if constexpr (DIM == 3) store_trace<T, DIM, P, 2>(v.trace[2], v.plane_size[2], d, slot2, psi);
-      }
-      T *dst = s_psi + w * 64 * NP + lane * NP;
-#pragma unroll
-      for (int i = 0; i < N; ++i) dst[natural_index<DIM, P>(i, o)] = psi[i];
-      if (more) {
-#pragma unroll
-        for (int f = 0; f < F; ++f) {
-          tx[f] = nx_[f];
-          ty[f] = ny_[f];
-          if constexpr (DIM == 3) tz[f] = nz_[f];
-        }
-        if constexpr (C::mfma) af = naf;
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int t = 0; t < NC; ++t) {
-      const int col = (t * W + w) * 64 + lane;
-      if (col < 64 * N) {
-        const int cl = col / N, i = col - cl * N;
-        for (int ww = 0; ww < W; ++ww) {
-          const int sl = r0 + ww;
-          if (sl >= ndir_o) break;
-          const int dd = dirs[sl];
-          const double pv = double(s_psi[ww * 64 * NP + cl * NP + i]);
-#pragma unroll
-          for (int m = 0; m < MM; ++m)
-            if (m < n_mom) acc[m][t] += d2m[m * n_dir + dd] * pv;
-        }
-      }
-    }
-    __syncthreads();
-  }
-
-#pragma unroll
-  for (int t = 0; t < NC; ++t) {
-    const int col = (t * W + w) * 64 + lane;
-    if (col < 64 * N) {
-      const int cl = col / N, i = col - cl * N;
-      if (cl < cnt) {
-        double *out = v.partial + long(o) * n_mom * n_dofs + long(s_cell[cl]) * N + i;
-#pragma unroll
-        for (int m = 0; m < MM; ++m)
-          if (m < n_mom) out[long(m) * n_dofs] = acc[m][t];
-      }
-    }
-  }
-}
+#include "sweep_level.inc"
 
 // Rearranges the canonical row-major inverses of FP32 Q2 hexahedra into the
 // A-operand layout of solve_mfma27: out[md][lane][kk] = A[lane%32][2kk+lane/32].
@@ -577,16 +302,21 @@ LevelLauncher pick(int dim, int p, int mm, int waves) {
     if (dim == 3 && p == 2) return pick_mm<T, 3, 2, 4>(mm);
   } else if (waves == 8) {
     if (dim == 3 && p == 2 && mm <= 4) return pick_mm<T, 3, 2, 8>(mm);
+  } else if (waves == 12) {
+    if (dim == 3 && p == 2 && mm <= 4) return pick_mm<T, 3, 2, 12>(mm);
   }
   return nullptr;
 }
 
 template <typename T>
 size_t lds_need(int dim, int p, int mm, int waves) {
-  const int P1 = p + 1, N = dim == 2 ? P1 * P1 : P1 * P1 * P1, NP = N | 1;
+  const int P1 = p + 1, N = dim == 2 ? P1 * P1 : P1 * P1 * P1;
+  const int NP = sizeof(T) == 4 ? (N + 3) & ~3 : (N + 1) & ~1;
   const size_t psi = size_t(waves) * 64 * NP * sizeof(T);
   const size_t src = size_t(mm) * 64 * NP * sizeof(T);
-  return psi + (psi + src <= 98304 ? src : 0) + 256;
+  const size_t tab =
+      size_t(kMaxSlots) * (2 * sizeof(int) + (mm + 4) * sizeof(T) + mm * sizeof(double));
+  return psi + (psi + src + tab <= 120 * 1024 ? src : 0) + tab + 256;
 }
 
 template <typename T>
@@ -649,17 +379,21 @@ SweepView Plan::view() const {
   v.n_dir = n_dir;
   v.n_mom = n_mom;
   v.n_dofs = n_dofs;
+  v.n_cls = n_cls;
+  v.slot_cls = d_slot_cls;
   v.ainv = d_ainv;
   v.ainv_mfma = d_ainv_mfma;
   v.S = d_S;
   v.S0 = nullptr;
-  v.m2d = d_m2d;
-  v.d2m = d_d2m;
-  v.omega_abs = d_omega_abs;
-  v.dir_oct = d_dir_oct;
+  v.slot_m2d = d_slot_m2d;
+  v.slot_d2m = d_slot_d2m;
+  v.slot_c = d_slot_c;
+  for (int k = 0; k < 16; ++k) {
+    v.m1f[k] = float(m1[k]);
+    v.m1d[k] = m1[k];
+  }
   v.oct_dirs = d_oct_dirs;
   v.oct_off = d_oct_off;
-  v.face_mass = d_face_mass;
   v.partial = d_partial;
   v.chunks = d_chunks;
   v.cells = d_cells;
@@ -685,22 +419,22 @@ void launch_build_inverses(Plan &pl) {
   OU_HIP(hipMalloc(&dM, M.size() * 8));
   OU_HIP(hipMalloc(&dG, G.size() * 8));
   OU_HIP(hipMalloc(&dE, E.size() * 8));
-  OU_HIP(hipMalloc(&dom, pl.omega.size() * 8));
+  OU_HIP(hipMalloc(&dom, pl.cls_omega.size() * 8));
   OU_HIP(hipMalloc(&dst, pl.sigma_t.size() * 8));
   OU_HIP(hipMemcpy(dM, M.data(), M.size() * 8, hipMemcpyHostToDevice));
   OU_HIP(hipMemcpy(dG, G.data(), G.size() * 8, hipMemcpyHostToDevice));
   OU_HIP(hipMemcpy(dE, E.data(), E.size() * 8, hipMemcpyHostToDevice));
-  OU_HIP(hipMemcpy(dom, pl.omega.data(), pl.omega.size() * 8, hipMemcpyHostToDevice));
+  OU_HIP(hipMemcpy(dom, pl.cls_omega.data(), pl.cls_omega.size() * 8, hipMemcpyHostToDevice));
   OU_HIP(hipMemcpy(dst, pl.sigma_t.data(), pl.sigma_t.size() * 8, hipMemcpyHostToDevice));
   const size_t lds = (size_t(N) * N + size_t(N) * 64) * sizeof(double);
-  const unsigned blocks = unsigned(pl.n_mat * pl.n_dir);
+  const unsigned blocks = unsigned(pl.n_mat * pl.n_cls);
   if (pl.precision == OU_FP32)
     hipLaunchKernelGGL(k_build_inverses<float>, dim3(blocks), dim3(64), lds, pl.stream, N,
-                       pl.dim, pl.p, pl.n_dir, dom, dst, dM, dG, dE,
+                       pl.dim, pl.p, pl.n_cls, dom, dst, dM, dG, dE,
                        static_cast<float *>(pl.d_ainv));
   else
     hipLaunchKernelGGL(k_build_inverses<double>, dim3(blocks), dim3(64), lds, pl.stream, N,
-                       pl.dim, pl.p, pl.n_dir, dom, dst, dM, dG, dE,
+                       pl.dim, pl.p, pl.n_cls, dom, dst, dM, dG, dE,
                        static_cast<double *>(pl.d_ainv));
   OU_HIP(hipGetLastError());
   if (pl.use_mfma)
