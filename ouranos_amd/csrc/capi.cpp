@@ -353,7 +353,15 @@ int ou_memcpy_dtoh(void *dst, const void *src, size_t bytes) {
 
 int ou_memcpy_dtod(void *dst, const void *src, size_t bytes) {
   return guarded([&] {
-    OU_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+    // A plain copy kernel (not a DMA engine), so that its memory traffic is
+    // visible to the same counters as the operator's kernels.
+    if (bytes % 8 == 0) {
+      ou::vec_copy(long(bytes / 8), static_cast<const double *>(src), static_cast<double *>(dst),
+                   nullptr);
+      OU_HIP(hipDeviceSynchronize());
+    } else {
+      OU_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+    }
     return int(OU_OK);
   });
 }

@@ -164,7 +164,7 @@ Plan *create_plan(const ou_problem_desc *desc) {
       OU_HIP(hipDeviceGetAttribute(&p.max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
     }
     p.use_mfma = p.precision == OU_FP32 && p.N == 27;
-    p.waves = p.use_mfma ? 8 : 4;
+    p.waves = p.use_mfma ? 12 : 4;
     if (const char *env = std::getenv("OURANOS_SWEEP_WAVES")) p.waves = std::atoi(env);
     if (!sweep_supported(p.dim, p.p, p.mm, p.precision, p.waves, p.max_lds)) p.waves = 4;
     if (!sweep_supported(p.dim, p.p, p.mm, p.precision, p.waves, p.max_lds))
@@ -254,11 +254,31 @@ Plan *create_plan(const ou_problem_desc *desc) {
         }
       }
     }
+    // Order each level's chunks so that all chunks of one material (every
+    // octant uses the same inverse classes) land on the same XCD: workgroup i
+    // of a launch runs on XCD i mod 8, each XCD with its own L2.  Chunks are
+    // bucketed by material mod 8 and dealt out so position p draws from
+    // bucket p mod 8 (from the fullest bucket once that one is empty).
+    constexpr int kXcds = 8;
     std::vector<Chunk> chunks;
     p.level_chunk_off.assign(p.n_levels + 1, 0);
     for (int l = 0; l < p.n_levels; ++l) {
       p.level_chunk_off[l] = int(chunks.size());
-      chunks.insert(chunks.end(), per_level[l].begin(), per_level[l].end());
+      std::vector<std::vector<Chunk>> xb(kXcds);
+      for (const Chunk &c : per_level[l]) xb[c.material % kXcds].push_back(c);
+      std::vector<size_t> next(kXcds, 0);
+      for (size_t pos = 0; pos < per_level[l].size(); ++pos) {
+        int b = int(pos % kXcds);
+        if (next[b] == xb[b].size()) {
+          size_t best = 0;
+          for (int q = 0; q < kXcds; ++q)
+            if (xb[q].size() - next[q] > best) {
+              best = xb[q].size() - next[q];
+              b = q;
+            }
+        }
+        chunks.push_back(xb[b][next[b]++]);
+      }
     }
     p.level_chunk_off[p.n_levels] = int(chunks.size());
     p.n_chunks = long(chunks.size());

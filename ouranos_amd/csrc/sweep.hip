@@ -304,6 +304,8 @@ LevelLauncher pick(int dim, int p, int mm, int waves) {
     if (dim == 3 && p == 2 && mm <= 4) return pick_mm<T, 3, 2, 8>(mm);
   } else if (waves == 12) {
     if (dim == 3 && p == 2 && mm <= 4) return pick_mm<T, 3, 2, 12>(mm);
+  } else if (waves == 6) {
+    if (dim == 3 && p == 2 && mm <= 4) return pick_mm<T, 3, 2, 6>(mm);
   }
   return nullptr;
 }
