@@ -45,6 +45,18 @@ def workload(name):
     raise SystemExit(f"unknown workload {name}")
 
 
+def measured_traffic(workload_name, precision):
+    """HBM traffic of one sweep (bytes, all level launches) from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload, if present."""
+    path = os.path.join(ROOT, "profiles", f"pmc_k_sweep_level_{workload_name}_{precision}.md")
+    if not os.path.exists(path):
+        return None, None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_sweep:"):
+            return int(line.split(":")[1]), os.path.relpath(path, ROOT)
+    return None, None
+
+
 def cpu_baseline(spec, seconds_budget, threads):
     """Times the FP64 oracle sweep (plain C restatement of
     RadiativeTransfer::sweep, one cell per task, Crout LU per cell) on a slab
@@ -131,6 +143,7 @@ def main():
     sweep_ms = tm["ms_sweep_kernel"] / a.steps
     alg_bytes = spec.algorithmic_bytes_per_sweep()
     achieved = alg_bytes / (sweep_ms / 1000.0) / 1e9
+    traffic, traffic_src = measured_traffic(spec.name, a.precision)
     line = {
         "metric": METRIC, "value": value, "unit": "DoF-angle updates/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": ms_step,
@@ -141,7 +154,8 @@ def main():
                    "dofs": spec.n_dofs, "updates_per_sweep": updates,
                    "accumulation": "fp64 moments", "seed": spec.meta.get("seed")},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": traffic_src,
                      "kernel": "k_sweep_level (all level launches of one sweep)",
                      "algorithmic_bytes_per_sweep": alg_bytes,
                      "sweep_kernel_ms_per_step": sweep_ms,
