@@ -177,14 +177,101 @@ typedef struct {
 } ou_solver_opts;
 
 typedef struct {
-  int iterations;      /* operator applications used */
+  int iterations;      /* operator applications used (FP32 ones for ou_solve_refined) */
   double rel_residual; /* final relative residual (FP64 test) */
   double seconds;      /* wall time of the solve */
   int converged;
+  int fp64_applications; /* ou_solve_refined: FP64 residual evaluations */
+  int preconditioner_applications;
 } ou_solve_info;
 
 int ou_solve(ou_plan *plan, const ou_solver_opts *opts, double *phi,
              ou_solve_info *info);
+
+/* ------------------------------------------------------------------------ */
+/* Learned right preconditioner (no reference counterpart: the reference    */
+/* solves with PreconditionIdentity, src/ouranos.cc:161).                   */
+/*                                                                          */
+/* A 3-level convolutional U-Net (32 base channels, BF16 weights and        */
+/* activations, FP32 accumulation on MFMA) on a G^3 coarse cell grid.  Per  */
+/* coarse cell it reads 12 FP32 features: r_hat[8] (the residual's octant   */
+/* means, see ou_coarse_features, divided by their RMS), tau_hat,           */
+/* kappa_hat, g and one unused slot; it predicts the moment-0 correction at */
+/* the cell's 8 corners (corner k = kx + 2 ky + 4 kz):                      */
+/* pred = gain (skip r_hat + (N(+r_hat) - N(-r_hat)) / 2), with gain = g or */
+/* 1 and skip = 1 or 0 by the network's mode.  Feature and target arrays    */
+/* are [batch][G^3][12] and [batch][G^3][8], cell index x fastest.          */
+/* ------------------------------------------------------------------------ */
+typedef struct ou_unet ou_unet;
+
+int ou_unet_create(int grid, int max_batch, unsigned long long seed, ou_unet **net);
+void ou_unet_destroy(ou_unet *net);
+long ou_unet_n_params(const ou_unet *net);
+int ou_unet_grid(const ou_unet *net);
+/* mode bit 0: add the skip r_hat; bit 1: scale by the prior g (default 0). */
+int ou_unet_set_mode(ou_unet *net, int mode);
+int ou_unet_mode(const ou_unet *net);
+/* FP32 master parameters (layout: per convolution W[27][cout][cin], bias[cout];
+ * head w[8][32], b[8]). */
+int ou_unet_get_params(const ou_unet *net, float *params);
+int ou_unet_set_params(ou_unet *net, const float *params);
+/* Host arrays in, host arrays out. */
+int ou_unet_predict(ou_unet *net, const float *features, int batch, float *pred);
+/* loss = mean_b |pred_b - target_b|^2 / |target_b|^2 and its gradient with
+ * respect to the parameters (grad may be NULL). */
+int ou_unet_loss_grad(ou_unet *net, const float *features, const float *target, int batch,
+                      double *loss, float *grad);
+/* One Adam step (lr, beta1 0.9, beta2 0.999) on that loss; *loss is the loss
+ * before the update. */
+int ou_unet_train_step(ou_unet *net, const float *features, const float *target, int batch,
+                       float lr, double *loss);
+
+/* Coarse view of a 3D n^3 plan on the G^3 grid (n a multiple of G):
+ *   coarse[C][k]    = mean of the moment-0 nodal values of v in octant
+ *                     k = kx + 2 ky + 4 kz of coarse cell C (nodes on a
+ *                     mid-plane count half to each side; skipped if v or
+ *                     coarse is NULL),
+ *   material[C][3]  = tau_hat, kappa_hat, g (skipped if NULL).
+ * v is a host vector of ou_domain_size doubles. */
+int ou_coarse_features(ou_plan *plan, int grid, const double *v, double *coarse,
+                       float *material);
+
+/* The preconditioner's prolongation: z[moment 0] += P(corners), where the
+ * corner values corners[C][k] (k = kx + 2 ky + 4 kz) are averaged over the
+ * coarse cells sharing each coarse vertex and interpolated trilinearly to
+ * every fine node.  z is a host vector of ou_domain_size doubles. */
+int ou_coarse_prolong(ou_plan *plan, int grid, const float *corners, double *z);
+
+/* Training pairs from the operator's own error propagation: starting from
+ * e_0 (host vector, or NULL for e_0 = the source-driven right-hand side
+ * D L^-1 (q + inflow)), runs e_{k+1} = e_k - A e_k for n_steps steps on the
+ * GPU (FP64 as the plan's precision) and, for every k with bit k of
+ * record_mask set, stores the coarse pair (R A e_k, R e_k) into
+ * r_out[j][G^3][8], e_out[j][G^3][8] (j counts the recorded steps).  Returns the
+ * number of recorded pairs, or a negative ou_status. */
+int ou_learned_pairs(ou_plan *plan, int grid, const double *e0, int n_steps,
+                     unsigned long long record_mask, double *r_out, double *e_out);
+
+/* Training pairs from plain GMRES(restart) on the source problem: every
+ * Arnoldi vector v_j comes with A v_j, so (R A v_j, R v_j) is an exact
+ * (residual, correction) pair of the kind FGMRES hands its preconditioner.
+ * Runs n_steps applications and records every stride-th pair, at most
+ * max_records, into r_out/e_out [j][G^3][8].  Returns the number recorded. */
+int ou_learned_pairs_krylov(ou_plan *plan, int grid, int n_steps, int restart, int stride,
+                            int max_records, double *r_out, double *e_out);
+
+/* FGMRES(restart) on the plan's operator, right-preconditioned by the
+ * network (opts->solver must be OU_SOLVER_FGMRES; net NULL = unpreconditioned). */
+int ou_solve_learned(ou_plan *plan, ou_unet *net, const ou_solver_opts *opts, double *phi,
+                     ou_solve_info *info);
+
+/* Mixed precision: FP64 residuals on plan64, inner FGMRES(restart) solves of
+ * the correction on the FP32 plan32 (same problem) to inner_tol relative,
+ * right-preconditioned by net (NULL = plain GMRES).  opts->tol is the FP64
+ * relative residual to reach; opts->max_it bounds the FP32 applications. */
+int ou_solve_refined(ou_plan *plan64, ou_plan *plan32, ou_unet *net,
+                     const ou_solver_opts *opts, double inner_tol, double *phi,
+                     ou_solve_info *info);
 
 /* ------------------------------------------------------------------------ */
 /* Device memory and timing plumbing (for callers without a HIP runtime of  */

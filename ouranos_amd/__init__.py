@@ -21,7 +21,8 @@ import numpy as np
 __all__ = [
     "Quadrature", "RadiativeTransfer", "fe_matrices", "OuError", "lib", "LIB_PATH",
     "FP64", "FP32", "LS", "GLC", "PRODUCT_GLC", "SI", "GMRES", "BICGSTAB", "FGMRES",
-    "BC_VACUUM", "BC_ISOTROPIC", "BC_MOST_NORMAL", "DeviceBuffer", "device_name",
+    "BC_VACUUM", "BC_ISOTROPIC", "BC_MOST_NORMAL", "DeviceBuffer", "device_name", "UNet",
+    "solve_refined",
 ]
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libouranos_amd.so")
@@ -65,7 +66,15 @@ class SolverOpts(ctypes.Structure):
 
 class SolveInfo(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int), ("rel_residual", ctypes.c_double),
-                ("seconds", ctypes.c_double), ("converged", ctypes.c_int)]
+                ("seconds", ctypes.c_double), ("converged", ctypes.c_int),
+                ("fp64_applications", ctypes.c_int),
+                ("preconditioner_applications", ctypes.c_int)]
+
+    def as_dict(self):
+        return {"iterations": self.iterations, "rel_residual": self.rel_residual,
+                "seconds": self.seconds, "converged": bool(self.converged),
+                "fp64_applications": self.fp64_applications,
+                "preconditioner_applications": self.preconditioner_applications}
 
 
 class Timing(ctypes.Structure):
@@ -102,6 +111,31 @@ def lib():
         L.ou_scatter_source_device.argtypes = [_vp, _vp, _vp]
         L.ou_time_apply.argtypes = [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int,
                                     ctypes.POINTER(Timing)]
+        _fp = ctypes.POINTER(ctypes.c_float)
+        L.ou_unet_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_ulonglong,
+                                     ctypes.POINTER(_vp)]
+        L.ou_unet_destroy.argtypes = [_vp]
+        L.ou_unet_destroy.restype = None
+        L.ou_unet_n_params.argtypes = [_vp]
+        L.ou_unet_n_params.restype = ctypes.c_long
+        L.ou_unet_grid.argtypes = [_vp]
+        L.ou_unet_set_mode.argtypes = [_vp, ctypes.c_int]
+        L.ou_unet_mode.argtypes = [_vp]
+        L.ou_unet_get_params.argtypes = [_vp, _fp]
+        L.ou_unet_set_params.argtypes = [_vp, _fp]
+        L.ou_unet_predict.argtypes = [_vp, _fp, ctypes.c_int, _fp]
+        L.ou_unet_loss_grad.argtypes = [_vp, _fp, _fp, ctypes.c_int, _dp, _fp]
+        L.ou_unet_train_step.argtypes = [_vp, _fp, _fp, ctypes.c_int, ctypes.c_float, _dp]
+        L.ou_coarse_features.argtypes = [_vp, ctypes.c_int, _dp, _dp, _fp]
+        L.ou_coarse_prolong.argtypes = [_vp, ctypes.c_int, _fp, _dp]
+        L.ou_learned_pairs.argtypes = [_vp, ctypes.c_int, _dp, ctypes.c_int,
+                                       ctypes.c_ulonglong, _dp, _dp]
+        L.ou_learned_pairs_krylov.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                              ctypes.c_int, ctypes.c_int, _dp, _dp]
+        L.ou_solve_learned.argtypes = [_vp, _vp, ctypes.POINTER(SolverOpts), _dp,
+                                       ctypes.POINTER(SolveInfo)]
+        L.ou_solve_refined.argtypes = [_vp, _vp, _vp, ctypes.POINTER(SolverOpts),
+                                       ctypes.c_double, _dp, ctypes.POINTER(SolveInfo)]
         _lib = L
     return _lib
 
@@ -329,8 +363,79 @@ class RadiativeTransfer:
         code = lib().ou_solve(self._plan, ctypes.byref(opts), _d(phi), ctypes.byref(info))
         if code != 0 and (raise_on_fail or code != -6):
             _check(code)
-        return phi, {"iterations": info.iterations, "rel_residual": info.rel_residual,
-                     "seconds": info.seconds, "converged": bool(info.converged)}
+        return phi, info.as_dict()
+
+    def solve_learned(self, net=None, tol=1e-8, max_it=1000, restart=30, phi0=None,
+                      verbose=False, raise_on_fail=True):
+        """FGMRES(restart), right-preconditioned by the U-Net `net` (None: plain)."""
+        phi = np.zeros(self.n) if phi0 is None else np.array(phi0, dtype=np.float64)
+        opts = SolverOpts(FGMRES, float(tol), int(max_it), int(restart), int(verbose))
+        info = SolveInfo()
+        code = lib().ou_solve_learned(self._plan, net._net if net is not None else None,
+                                      ctypes.byref(opts), _d(phi), ctypes.byref(info))
+        if code != 0 and (raise_on_fail or code != -6):
+            _check(code)
+        return phi, info.as_dict()
+
+    # --- learned-preconditioner data -----------------------------------------
+    def coarse_features(self, grid, v=None):
+        """(octant means [G^3][8] of v's moment 0 or None, material features [G^3][3])."""
+        P0 = grid ** 3
+        mat = np.empty(P0 * 3, dtype=np.float32)
+        coarse = None
+        if v is not None:
+            v = np.ascontiguousarray(v, dtype=np.float64)
+            if v.size != self.n:
+                raise OuError(-2, f"v has {v.size} entries, operator expects {self.n}")
+            coarse = np.empty((P0, 8))
+            _check(lib().ou_coarse_features(self._plan, int(grid), _d(v), _d(coarse),
+                                            mat.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        else:
+            _check(lib().ou_coarse_features(self._plan, int(grid), None, None,
+                                            mat.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        return coarse, mat.reshape(P0, 3)
+
+    def coarse_prolong(self, grid, corners, z=None):
+        """z + P(corners): corner values [G^3][8] -> continuous moment-0 field."""
+        z = np.zeros(self.n) if z is None else np.array(z, dtype=np.float64)
+        c = np.ascontiguousarray(corners, dtype=np.float32)
+        if c.size != 8 * grid ** 3:
+            raise OuError(-2, f"corners must hold 8 * grid^3 values, got {c.size}")
+        _check(lib().ou_coarse_prolong(self._plan, int(grid),
+                                       c.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), _d(z)))
+        return z
+
+    def learned_pairs_krylov(self, grid, n_steps, stride=1, max_records=64, restart=30):
+        """Coarse pairs (R A v_j, R v_j) of plain GMRES's Arnoldi vectors."""
+        P0 = grid ** 3
+        r = np.empty((max_records, P0, 8))
+        e = np.empty((max_records, P0, 8))
+        got = lib().ou_learned_pairs_krylov(self._plan, int(grid), int(n_steps), int(restart),
+                                            int(stride), int(max_records), _d(r), _d(e))
+        if got < 0:
+            _check(got)
+        return r[:got], e[:got]
+
+    def learned_pairs(self, grid, n_steps, record, e0=None):
+        """Coarse pairs (R A e_k, R e_k) of e_{k+1} = e_k - A e_k for k in `record`."""
+        mask = 0
+        for k in record:
+            if not 0 <= k < n_steps:
+                raise ValueError("record steps must lie in [0, n_steps)")
+            mask |= 1 << int(k)
+        n_rec = bin(mask).count("1")
+        P0 = grid ** 3
+        r = np.empty((n_rec, P0, 8))
+        e = np.empty((n_rec, P0, 8))
+        if e0 is not None:
+            e0 = np.ascontiguousarray(e0, dtype=np.float64)
+            if e0.size != self.n:
+                raise OuError(-2, f"e0 has {e0.size} entries, operator expects {self.n}")
+        got = lib().ou_learned_pairs(self._plan, int(grid), _d(e0) if e0 is not None else None,
+                                     int(n_steps), ctypes.c_ulonglong(mask), _d(r), _d(e))
+        if got < 0:
+            _check(got)
+        return r[:got], e[:got]
 
     # --- device API --------------------------------------------------------------
     def apply_device(self, d_x, d_y):
@@ -354,3 +459,99 @@ class RadiativeTransfer:
             self.close()
         except Exception:
             pass
+
+
+class UNet:
+    """The learned right preconditioner's network (ou_unet_*): a 3-level
+    convolutional U-Net with 32 base channels on a grid^3 coarse cell grid,
+    BF16 weights/activations with FP32 accumulation on MFMA.  Features are
+    float32 [batch][grid^3][12] (r_hat[8], tau_hat, kappa_hat, g, unused),
+    targets and predictions [batch][grid^3][8] (corner values)."""
+
+    N_FEATURES = 12
+    N_OUT = 8
+
+    def __init__(self, grid=32, max_batch=8, seed=0):
+        self._net = _vp()
+        _check(lib().ou_unet_create(int(grid), int(max_batch), ctypes.c_ulonglong(seed),
+                                    ctypes.byref(self._net)))
+        self.grid = int(grid)
+        self.max_batch = int(max_batch)
+        self.n_params = int(lib().ou_unet_n_params(self._net))
+
+    @staticmethod
+    def _f(a):
+        return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+    def _feat(self, feat):
+        feat = np.ascontiguousarray(feat, dtype=np.float32).reshape(
+            -1, self.grid ** 3, self.N_FEATURES)
+        return feat, feat.shape[0]
+
+    @property
+    def mode(self):
+        """Prediction form: bit 0 adds the skip r_hat, bit 1 scales by g."""
+        return int(lib().ou_unet_mode(self._net))
+
+    @mode.setter
+    def mode(self, m):
+        _check(lib().ou_unet_set_mode(self._net, int(m)))
+
+    def get_params(self):
+        p = np.empty(self.n_params, dtype=np.float32)
+        _check(lib().ou_unet_get_params(self._net, self._f(p)))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        if p.size != self.n_params:
+            raise OuError(-2, f"expected {self.n_params} parameters, got {p.size}")
+        _check(lib().ou_unet_set_params(self._net, self._f(p)))
+
+    def predict(self, feat):
+        feat, B = self._feat(feat)
+        out = np.empty((B, self.grid ** 3, self.N_OUT), dtype=np.float32)
+        _check(lib().ou_unet_predict(self._net, self._f(feat), B, self._f(out)))
+        return out
+
+    def loss_grad(self, feat, target):
+        feat, B = self._feat(feat)
+        target = np.ascontiguousarray(target, dtype=np.float32).reshape(B, -1)
+        loss = ctypes.c_double()
+        grad = np.empty(self.n_params, dtype=np.float32)
+        _check(lib().ou_unet_loss_grad(self._net, self._f(feat), self._f(target), B,
+                                       ctypes.byref(loss), self._f(grad)))
+        return loss.value, grad
+
+    def train_step(self, feat, target, lr):
+        feat, B = self._feat(feat)
+        target = np.ascontiguousarray(target, dtype=np.float32).reshape(B, -1)
+        loss = ctypes.c_double()
+        _check(lib().ou_unet_train_step(self._net, self._f(feat), self._f(target), B,
+                                        ctypes.c_float(lr), ctypes.byref(loss)))
+        return loss.value
+
+    def close(self):
+        if getattr(self, "_net", None):
+            lib().ou_unet_destroy(self._net)
+            self._net = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def solve_refined(op64, op32, net=None, tol=1e-8, max_it=5000, restart=30, inner_tol=1e-3,
+                  phi0=None, verbose=False, raise_on_fail=True):
+    """FP64-residual refinement around FGMRES on the FP32 operator (ou_solve_refined)."""
+    phi = np.zeros(op64.n) if phi0 is None else np.array(phi0, dtype=np.float64)
+    opts = SolverOpts(FGMRES, float(tol), int(max_it), int(restart), int(verbose))
+    info = SolveInfo()
+    code = lib().ou_solve_refined(op64._plan, op32._plan, net._net if net is not None else None,
+                                  ctypes.byref(opts), ctypes.c_double(inner_tol), _d(phi),
+                                  ctypes.byref(info))
+    if code != 0 and (raise_on_fail or code != -6):
+        _check(code)
+    return phi, info.as_dict()

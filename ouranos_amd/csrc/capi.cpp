@@ -6,12 +6,24 @@
 #include <string>
 
 #include "fe.hpp"
+#include "learned.hpp"
 #include "quadrature.hpp"
 #include "radiative_transfer.hpp"
+#include "unet.hpp"
 #include "vec.hpp"
 
 struct ou_plan {
   std::unique_ptr<ou::RadiativeTransfer> rt;
+};
+
+// The network plus device staging for the host-array entry points.
+struct ou_unet {
+  std::unique_ptr<ou::UNet> net;
+  float *d_feat = nullptr, *d_target = nullptr, *d_pred = nullptr;
+  ~ou_unet() {
+    for (float *p : {d_feat, d_target, d_pred})
+      if (p) (void)hipFree(p);
+  }
 };
 
 struct ou_quadrature {
@@ -317,6 +329,8 @@ int ou_solve(ou_plan *plan, const ou_solver_opts *opts, double *phi, ou_solve_in
       info->rel_residual = r.rel_residual;
       info->seconds = secs;
       info->converged = r.converged ? 1 : 0;
+      info->fp64_applications = pl.precision == OU_FP64 ? r.iterations : 0;
+      info->preconditioner_applications = 0;
     }
     return r.converged ? int(OU_OK) : int(OU_ERR_NOT_CONVERGED);
   });
@@ -419,6 +433,358 @@ int ou_time_apply(ou_plan *plan, const double *d_x, double *d_y, int warmup, int
     timing->n_sweep_launches = launches;
     timing->applications = steps;
     return int(OU_OK);
+  });
+}
+
+// ---------------------------------------------------------------------------
+// Learned preconditioner
+// ---------------------------------------------------------------------------
+
+int ou_unet_create(int grid, int max_batch, unsigned long long seed, ou_unet **net) {
+  return guarded([&] {
+    if (!net) ou::fail(OU_ERR_ARG, "null output pointer");
+    std::unique_ptr<ou_unet> u(new ou_unet());
+    u->net.reset(new ou::UNet(grid, max_batch, seed));
+    const size_t p0 = size_t(grid) * grid * grid * max_batch;
+    OU_HIP(hipMalloc(&u->d_feat, p0 * ou::kUnetFeatures * 4));
+    OU_HIP(hipMalloc(&u->d_target, p0 * ou::kUnetOut * 4));
+    OU_HIP(hipMalloc(&u->d_pred, p0 * ou::kUnetOut * 4));
+    *net = u.release();
+    return int(OU_OK);
+  });
+}
+
+void ou_unet_destroy(ou_unet *net) { delete net; }
+
+long ou_unet_n_params(const ou_unet *net) { return net && net->net ? net->net->n_params() : 0; }
+
+int ou_unet_grid(const ou_unet *net) { return net && net->net ? net->net->grid() : 0; }
+
+int ou_unet_set_mode(ou_unet *net, int mode) {
+  return guarded([&] {
+    if (!net || !net->net) ou::fail(OU_ERR_ARG, "null network");
+    net->net->set_mode(mode);
+    return int(OU_OK);
+  });
+}
+
+int ou_unet_mode(const ou_unet *net) { return net && net->net ? net->net->mode() : -1; }
+
+int ou_unet_get_params(const ou_unet *net, float *params) {
+  return guarded([&] {
+    if (!net || !params) ou::fail(OU_ERR_ARG, "null argument");
+    net->net->get_params(params);
+    return int(OU_OK);
+  });
+}
+
+int ou_unet_set_params(ou_unet *net, const float *params) {
+  return guarded([&] {
+    if (!net || !params) ou::fail(OU_ERR_ARG, "null argument");
+    const long n = net->net->n_params();
+    for (long i = 0; i < n; ++i)
+      if (!std::isfinite(params[i])) ou::fail(OU_ERR_NONFINITE, "non-finite parameter");
+    net->net->set_params(params);
+    return int(OU_OK);
+  });
+}
+
+namespace {
+
+void unet_upload(ou_unet *net, const float *features, const float *target, int batch) {
+  if (!net || !features) ou::fail(OU_ERR_ARG, "null argument");
+  if (batch < 1 || batch > net->net->max_batch())
+    ou::fail(OU_ERR_SIZE, "batch exceeds the network's maximum batch");
+  const size_t p0 = size_t(net->net->grid()) * net->net->grid() * net->net->grid() * batch;
+  for (size_t i = 0; i < p0 * ou::kUnetFeatures; ++i)
+    if (!std::isfinite(features[i])) ou::fail(OU_ERR_NONFINITE, "non-finite feature");
+  OU_HIP(hipMemcpy(net->d_feat, features, p0 * ou::kUnetFeatures * 4, hipMemcpyHostToDevice));
+  if (target) {
+    for (size_t i = 0; i < p0 * ou::kUnetOut; ++i)
+      if (!std::isfinite(target[i])) ou::fail(OU_ERR_NONFINITE, "non-finite target");
+    OU_HIP(hipMemcpy(net->d_target, target, p0 * ou::kUnetOut * 4, hipMemcpyHostToDevice));
+  }
+}
+
+}  // namespace
+
+int ou_unet_predict(ou_unet *net, const float *features, int batch, float *pred) {
+  return guarded([&] {
+    if (!pred) ou::fail(OU_ERR_ARG, "null output");
+    unet_upload(net, features, nullptr, batch);
+    net->net->predict(net->d_feat, batch, net->d_pred, nullptr);
+    const size_t p0 = size_t(net->net->grid()) * net->net->grid() * net->net->grid() * batch;
+    OU_HIP(hipMemcpy(pred, net->d_pred, p0 * ou::kUnetOut * 4, hipMemcpyDeviceToHost));
+    return int(OU_OK);
+  });
+}
+
+int ou_unet_loss_grad(ou_unet *net, const float *features, const float *target, int batch,
+                      double *loss, float *grad) {
+  return guarded([&] {
+    if (!target || !loss) ou::fail(OU_ERR_ARG, "null argument");
+    unet_upload(net, features, target, batch);
+    net->net->loss_and_grad(net->d_feat, net->d_target, batch, nullptr);
+    OU_HIP(hipMemcpy(loss, net->net->d_loss(), 8, hipMemcpyDeviceToHost));
+    if (grad)
+      OU_HIP(hipMemcpy(grad, net->net->d_grad(), size_t(net->net->n_params()) * 4,
+                       hipMemcpyDeviceToHost));
+    return int(OU_OK);
+  });
+}
+
+int ou_unet_train_step(ou_unet *net, const float *features, const float *target, int batch,
+                       float lr, double *loss) {
+  return guarded([&] {
+    if (!target || !loss) ou::fail(OU_ERR_ARG, "null argument");
+    if (!(lr > 0.0f) || !std::isfinite(lr)) ou::fail(OU_ERR_ARG, "learning rate must be positive");
+    unet_upload(net, features, target, batch);
+    net->net->loss_and_grad(net->d_feat, net->d_target, batch, nullptr);
+    net->net->adam_step(lr, nullptr);
+    OU_HIP(hipMemcpy(loss, net->net->d_loss(), 8, hipMemcpyDeviceToHost));
+    return int(OU_OK);
+  });
+}
+
+int ou_coarse_features(ou_plan *plan, int grid, const double *v, double *coarse, float *material) {
+  return guarded([&] {
+    ou::Plan &pl = plan_of(plan);
+    ou::coarse_factor(pl, grid);
+    const size_t p0 = size_t(grid) * grid * grid;
+    if (v && coarse) {
+      check_finite_host(v, pl.vec_size);
+      ensure_staging(pl);
+      OU_HIP(hipMemcpy(pl.d_x, v, size_t(pl.vec_size) * 8, hipMemcpyHostToDevice));
+      ou::coarse_restrict(pl, grid, pl.d_x, pl.d_y, pl.stream);
+      OU_HIP(hipStreamSynchronize(pl.stream));
+      OU_HIP(hipMemcpy(coarse, pl.d_y, p0 * 8 * 8, hipMemcpyDeviceToHost));
+    }
+    if (material) {
+      const std::vector<float> m = ou::coarse_material_features(pl, grid);
+      std::memcpy(material, m.data(), m.size() * 4);
+    }
+    return int(OU_OK);
+  });
+}
+
+int ou_coarse_prolong(ou_plan *plan, int grid, const float *corners, double *z) {
+  return guarded([&] {
+    ou::Plan &pl = plan_of(plan);
+    ou::coarse_factor(pl, grid);
+    if (!corners || !z) ou::fail(OU_ERR_ARG, "null argument");
+    check_finite_host(z, pl.vec_size);
+    const size_t p0 = size_t(grid) * grid * grid;
+    for (size_t i = 0; i < p0 * 8; ++i)
+      if (!std::isfinite(corners[i])) ou::fail(OU_ERR_NONFINITE, "non-finite corner value");
+    ensure_staging(pl);
+    float *d_c = nullptr;
+    double *d_aux = nullptr;
+    struct Free {
+      float **a;
+      double **b;
+      ~Free() {
+        if (*a) (void)hipFree(*a);
+        if (*b) (void)hipFree(*b);
+      }
+    } guard{&d_c, &d_aux};
+    const size_t nv = size_t(grid + 1) * (grid + 1) * (grid + 1);
+    OU_HIP(hipMalloc(&d_c, p0 * 8 * 4));
+    OU_HIP(hipMalloc(&d_aux, (nv + 1) * 8));
+    OU_HIP(hipMemcpy(d_c, corners, p0 * 8 * 4, hipMemcpyHostToDevice));
+    const double dot = 8.0 * double(p0);  // scale s = 1
+    OU_HIP(hipMemcpy(d_aux + nv, &dot, 8, hipMemcpyHostToDevice));
+    OU_HIP(hipMemcpy(pl.d_x, z, size_t(pl.vec_size) * 8, hipMemcpyHostToDevice));
+    ou::coarse_prolong_add(pl, grid, d_c, d_aux + nv, d_aux, pl.d_x, pl.stream);
+    OU_HIP(hipStreamSynchronize(pl.stream));
+    OU_HIP(hipMemcpy(z, pl.d_x, size_t(pl.vec_size) * 8, hipMemcpyDeviceToHost));
+    return int(OU_OK);
+  });
+}
+
+int ou_learned_pairs(ou_plan *plan, int grid, const double *e0, int n_steps,
+                     unsigned long long record_mask, double *r_out, double *e_out) {
+  return guarded([&] {
+    ou::Plan &pl = plan_of(plan);
+    ou::RadiativeTransfer &rt = *plan->rt;
+    ou::coarse_factor(pl, grid);
+    if (n_steps < 1 || n_steps > 64) ou::fail(OU_ERR_ARG, "n_steps must be in [1, 64]");
+    if (!r_out || !e_out) ou::fail(OU_ERR_ARG, "null output");
+    const long n = pl.vec_size;
+    const size_t p0 = size_t(grid) * grid * grid;
+    hipStream_t s = pl.stream;
+    double *d_e = nullptr, *d_ae = nullptr, *d_c = nullptr;
+    struct Free {
+      double **p[3];
+      ~Free() {
+        for (double **q : p)
+          if (*q) (void)hipFree(*q);
+      }
+    } guard{{&d_e, &d_ae, &d_c}};
+    OU_HIP(hipMalloc(&d_e, size_t(n) * 8));
+    OU_HIP(hipMalloc(&d_ae, size_t(n) * 8));
+    OU_HIP(hipMalloc(&d_c, p0 * 8 * 8));
+    if (e0) {
+      check_finite_host(e0, n);
+      OU_HIP(hipMemcpy(d_e, e0, size_t(n) * 8, hipMemcpyHostToDevice));
+    } else {
+      ou::build_rhs(rt, d_e, s);
+    }
+    int rec = 0;
+    for (int k = 0; k < n_steps; ++k) {
+      rt.Apply(d_e, d_ae, s);
+      if ((record_mask >> k) & 1ull) {
+        ou::coarse_restrict(pl, grid, d_ae, d_c, s);
+        OU_HIP(hipStreamSynchronize(s));
+        OU_HIP(hipMemcpy(r_out + size_t(rec) * p0 * 8, d_c, p0 * 8 * 8, hipMemcpyDeviceToHost));
+        ou::coarse_restrict(pl, grid, d_e, d_c, s);
+        OU_HIP(hipStreamSynchronize(s));
+        OU_HIP(hipMemcpy(e_out + size_t(rec) * p0 * 8, d_c, p0 * 8 * 8, hipMemcpyDeviceToHost));
+        ++rec;
+      }
+      ou::vec_axpby(n, -1.0, d_ae, 1.0, d_e, s);
+    }
+    OU_HIP(hipStreamSynchronize(s));
+    return rec;
+  });
+}
+
+namespace {
+
+// Records the coarse pair (R A z_j, R z_j) of every stride-th Arnoldi vector.
+struct PairRecorder : ou::ArnoldiObserver {
+  const ou::Plan &pl;
+  int grid, stride, max_records, count = 0;
+  double *d_c, *r_out, *e_out;
+  PairRecorder(const ou::Plan &p, int g, int st, int mr, double *dc, double *r, double *e)
+      : pl(p), grid(g), stride(st), max_records(mr), d_c(dc), r_out(r), e_out(e) {}
+  void observe(int it, const double *d_z, const double *d_az, hipStream_t s) override {
+    if (it % stride || count >= max_records) return;
+    const size_t m = size_t(grid) * grid * grid * 8;
+    ou::coarse_restrict(pl, grid, d_az, d_c, s);
+    OU_HIP(hipStreamSynchronize(s));
+    OU_HIP(hipMemcpy(r_out + size_t(count) * m, d_c, m * 8, hipMemcpyDeviceToHost));
+    ou::coarse_restrict(pl, grid, d_z, d_c, s);
+    OU_HIP(hipStreamSynchronize(s));
+    OU_HIP(hipMemcpy(e_out + size_t(count) * m, d_c, m * 8, hipMemcpyDeviceToHost));
+    ++count;
+  }
+};
+
+}  // namespace
+
+int ou_learned_pairs_krylov(ou_plan *plan, int grid, int n_steps, int restart, int stride,
+                            int max_records, double *r_out, double *e_out) {
+  return guarded([&] {
+    ou::Plan &pl = plan_of(plan);
+    ou::RadiativeTransfer &rt = *plan->rt;
+    ou::coarse_factor(pl, grid);
+    if (n_steps < 1 || stride < 1 || max_records < 1) ou::fail(OU_ERR_ARG, "bad step counts");
+    if (!r_out || !e_out) ou::fail(OU_ERR_ARG, "null output");
+    const long n = pl.vec_size;
+    hipStream_t s = pl.stream;
+    double *d_b = nullptr, *d_x = nullptr, *d_c = nullptr;
+    struct Free {
+      double **p[3];
+      ~Free() {
+        for (double **q : p)
+          if (*q) (void)hipFree(*q);
+      }
+    } guard{{&d_b, &d_x, &d_c}};
+    OU_HIP(hipMalloc(&d_b, size_t(n) * 8));
+    OU_HIP(hipMalloc(&d_x, size_t(n) * 8));
+    OU_HIP(hipMalloc(&d_c, size_t(grid) * grid * grid * 8 * 8));
+    ou::build_rhs(rt, d_b, s);
+    ou::vec_fill(n, 0.0, d_x, s);
+    PairRecorder rec(pl, grid, stride, max_records, d_c, r_out, e_out);
+    ou::solve_gmres(rt, d_b, d_x, restart, 1e-300, n_steps, nullptr, false, s, &rec);
+    OU_HIP(hipStreamSynchronize(s));
+    return rec.count;
+  });
+}
+
+namespace {
+
+void fill_info(ou_solve_info *info, const ou::SolveResult &r, double secs, int fp64, int prec) {
+  if (!info) return;
+  info->iterations = r.iterations;
+  info->rel_residual = r.rel_residual;
+  info->seconds = secs;
+  info->converged = r.converged ? 1 : 0;
+  info->fp64_applications = fp64;
+  info->preconditioner_applications = prec;
+}
+
+}  // namespace
+
+int ou_solve_learned(ou_plan *plan, ou_unet *net, const ou_solver_opts *opts, double *phi,
+                     ou_solve_info *info) {
+  return guarded([&] {
+    ou::Plan &pl = plan_of(plan);
+    if (!opts || !phi) ou::fail(OU_ERR_ARG, "null argument");
+    if (opts->solver != OU_SOLVER_FGMRES) ou::fail(OU_ERR_ARG, "ou_solve_learned runs FGMRES");
+    if (!(opts->tol > 0.0) || opts->max_it < 1) ou::fail(OU_ERR_ARG, "bad tolerance or max_it");
+    if (net && !net->net) ou::fail(OU_ERR_ARG, "empty network");
+    check_finite_host(phi, pl.vec_size);
+    ou::RadiativeTransfer &rt = *plan->rt;
+    std::unique_ptr<ou::LearnedPreconditioner> prec;
+    if (net) prec.reset(new ou::LearnedPreconditioner(rt, *net->net));
+    const size_t bytes = size_t(pl.vec_size) * 8;
+    double *d_phi = nullptr, *d_rhs = nullptr;
+    struct Free {
+      double **a, **b;
+      ~Free() {
+        if (*a) (void)hipFree(*a);
+        if (*b) (void)hipFree(*b);
+      }
+    } guard{&d_phi, &d_rhs};
+    OU_HIP(hipMalloc(&d_phi, bytes));
+    OU_HIP(hipMalloc(&d_rhs, bytes));
+    OU_HIP(hipMemcpy(d_phi, phi, bytes, hipMemcpyHostToDevice));
+    hipStream_t s = pl.stream;
+    const auto t0 = std::chrono::steady_clock::now();
+    ou::build_rhs(rt, d_rhs, s);
+    const ou::SolveResult r = ou::solve_gmres(rt, d_rhs, d_phi, opts->restart, opts->tol,
+                                              opts->max_it, prec.get(), opts->verbose != 0, s);
+    const double secs =
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    OU_HIP(hipMemcpy(phi, d_phi, bytes, hipMemcpyDeviceToHost));
+    fill_info(info, r, secs, pl.precision == OU_FP64 ? r.iterations : 0,
+              prec ? prec->applications() : 0);
+    return r.converged ? int(OU_OK) : int(OU_ERR_NOT_CONVERGED);
+  });
+}
+
+int ou_solve_refined(ou_plan *plan64, ou_plan *plan32, ou_unet *net, const ou_solver_opts *opts,
+                     double inner_tol, double *phi, ou_solve_info *info) {
+  return guarded([&] {
+    ou::Plan &p64 = plan_of(plan64);
+    plan_of(plan32);
+    if (!opts || !phi) ou::fail(OU_ERR_ARG, "null argument");
+    if (!(opts->tol > 0.0) || opts->max_it < 1 || !(inner_tol > 0.0) || !(inner_tol < 1.0))
+      ou::fail(OU_ERR_ARG, "bad tolerance or max_it");
+    if (net && !net->net) ou::fail(OU_ERR_ARG, "empty network");
+    check_finite_host(phi, p64.vec_size);
+    std::unique_ptr<ou::LearnedPreconditioner> prec;
+    if (net) prec.reset(new ou::LearnedPreconditioner(*plan32->rt, *net->net));
+    const size_t bytes = size_t(p64.vec_size) * 8;
+    double *d_phi = nullptr;
+    OU_HIP(hipMalloc(&d_phi, bytes));
+    struct Free {
+      double **a;
+      ~Free() {
+        if (*a) (void)hipFree(*a);
+      }
+    } guard{&d_phi};
+    OU_HIP(hipMemcpy(d_phi, phi, bytes, hipMemcpyHostToDevice));
+    const auto t0 = std::chrono::steady_clock::now();
+    const ou::RefinedResult r =
+        ou::solve_refined(*plan64->rt, *plan32->rt, prec.get(), d_phi, opts->tol, opts->max_it,
+                          opts->restart, inner_tol, opts->verbose != 0);
+    const double secs =
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    OU_HIP(hipMemcpy(phi, d_phi, bytes, hipMemcpyDeviceToHost));
+    fill_info(info, r.total, secs, r.outer, prec ? prec->applications() : 0);
+    return r.total.converged ? int(OU_OK) : int(OU_ERR_NOT_CONVERGED);
   });
 }
 

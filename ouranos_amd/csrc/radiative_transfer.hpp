@@ -82,9 +82,16 @@ SolveResult solve_si(RadiativeTransfer &rt, double *d_phi, double tol, int max_i
                      bool verbose);
 // Builds the right-hand side D L^-1 (q + inflow) (src/ouranos.cc:149-158).
 void build_rhs(RadiativeTransfer &rt, double *d_rhs, hipStream_t s);
+// Called after every Arnoldi application with (z_j, A z_j) on the device
+// (z_j = v_j without a preconditioner) and the iteration count.
+struct ArnoldiObserver {
+  virtual ~ArnoldiObserver() = default;
+  virtual void observe(int iteration, const double *d_z, const double *d_az, hipStream_t s) = 0;
+};
+
 SolveResult solve_gmres(Operator &op, const double *d_b, double *d_x, int restart,
                         double tol, int max_it, Preconditioner *prec, bool verbose,
-                        hipStream_t s);
+                        hipStream_t s, ArnoldiObserver *observer = nullptr);
 SolveResult solve_bicgstab(Operator &op, const double *d_b, double *d_x, double tol,
                            int max_it, bool verbose, hipStream_t s);
 

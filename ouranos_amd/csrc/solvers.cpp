@@ -85,7 +85,7 @@ SolveResult solve_si(RadiativeTransfer &rt, double *d_phi, double tol, int max_i
 
 SolveResult solve_gmres(Operator &op, const double *d_b, double *d_x, int restart,
                         double tol, int max_it, Preconditioner *prec, bool verbose,
-                        hipStream_t s) {
+                        hipStream_t s, ArnoldiObserver *observer) {
   const long n = op.DomainSize();
   const int m = restart > 0 ? restart : 30;
   if (m > 31) fail(OU_ERR_ARG, "restart length must be <= 31");
@@ -125,8 +125,10 @@ SolveResult solve_gmres(Operator &op, const double *d_b, double *d_x, int restar
         double *zj = Z->p + size_t(j) * n;
         prec->apply(vj, zj, s);
         op.Apply(zj, w.p, s);
+        if (observer) observer->observe(total, zj, w.p, s);
       } else {
         op.Apply(vj, w.p, s);
+        if (observer) observer->observe(total, vj, w.p, s);
       }
       ++total;
       // Classical Gram-Schmidt, applied twice (FP64).

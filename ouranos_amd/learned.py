@@ -1,0 +1,253 @@
+"""Training and evaluation of the learned FGMRES preconditioner.
+
+Data: seeded optically thick, highly scattering 3D instances (n^3 cells, DG Q1,
+level-symmetric S8, sigma_t log-uniform in [1, 1000] and c uniform in
+[0.99, 0.9999] on 4^3-cell blocks, block sources U[0, 1], vacuum).  Each
+instance contributes (residual, correction) pairs that the operator's own
+sweeps generate: along the error propagation e_{k+1} = e_k - A e_k of source
+iteration, (r, e) = (A e_k, e_k) is an exact pair, and the network learns the
+coarse part of the scattered correction e - r = e_{k+1}.  Two chains per
+instance: one from the source-driven right-hand side D L^-1 (q + inflow), one
+from a random cell field.
+
+Per coarse cell the network sees r_hat = R r / s (s = RMS of R r), tau_hat and
+kappa_hat, and predicts t = R (e - r) / s through the fixed prior g (see
+ouranos_amd/csrc/learned.hpp for the feature definitions).
+
+    python -m ouranos_amd.learned train --instances 2000 --out weights.npy
+    python -m ouranos_amd.learned evaluate --weights weights.npy --instances 200
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+import ouranos_amd as ou
+from ouranos_amd import problems as P
+
+TRAIN_SEED0 = 1_000_000   # instance i of the training set uses seed TRAIN_SEED0 + i
+EVAL_SEED0 = 2_000_000    # held-out instances
+RECORD = (0, 2, 6, 14)    # recorded steps k of each chain
+N_STEPS = 15
+
+
+def instance(seed, n=32, quad=None):
+    return P.thick_3d(n=n, block=4, seed=seed, quad=quad)
+
+
+def random_field(spec, seed):
+    """A random moment-0 field: per-cell normal values (higher moments zero)."""
+    rng = np.random.default_rng(seed)
+    e = np.zeros(spec.size)
+    cells = rng.normal(size=spec.n_cells)
+    e[:spec.n_dofs] = np.repeat(cells, spec.n_loc)
+    return e
+
+
+def features_of(rk, mat):
+    """Network features [G^3][12] of the octant-restricted residual rk [G^3][8]
+    and the material features mat [G^3][3]; returns (features, scale s)."""
+    s = float(np.sqrt(np.mean(rk * rk)))
+    f = np.zeros((rk.shape[0], ou.UNet.N_FEATURES), dtype=np.float32)
+    if s > 0:
+        f[:, 0:8] = rk / s
+    f[:, 8:11] = mat
+    return f, s
+
+
+KRYLOV_STEPS = 90   # plain GMRES(30) applications per instance
+KRYLOV_STRIDE = 6   # record every 6th Arnoldi pair
+
+
+def pairs_of(op, spec, grid, seed, sources=("si", "krylov")):
+    """Features [k][G^3][12] and targets [k][G^3][8] (float32) of one instance.
+    Sources: "si" = the error-propagation chains of source iteration from the
+    right-hand side and from a random field; "krylov" = the Arnoldi vectors
+    of plain GMRES(30) on the instance's source problem."""
+    _, mat = op.coarse_features(grid)
+    chunks = []
+    if "si" in sources:
+        for e0 in (None, random_field(spec, seed)):
+            chunks.append(op.learned_pairs(grid, N_STEPS, RECORD, e0=e0))
+    if "krylov" in sources:
+        chunks.append(op.learned_pairs_krylov(grid, KRYLOV_STEPS, stride=KRYLOV_STRIDE,
+                                              max_records=KRYLOV_STEPS // KRYLOV_STRIDE))
+    feats, targets = [], []
+    for r, e in chunks:
+        for rk, ek in zip(r, e):
+            f, s = features_of(rk, mat)
+            if not s > 0:
+                continue
+            feats.append(f)
+            targets.append(((ek - rk) / s).astype(np.float32))
+    return feats, targets
+
+
+def generate(n_instances, seed0, grid=32, n=32, log=print, sources=("si", "krylov")):
+    quad = ou.Quadrature(ou.LS, 3, sn=8)
+    feats, targets = [], []
+    t0 = time.perf_counter()
+    for i in range(n_instances):
+        spec = instance(seed0 + i, n=n, quad=quad)
+        op = spec.operator(ou.FP64)
+        f, t = pairs_of(op, spec, grid, seed0 + i, sources)
+        op.close()
+        feats += f
+        targets += t
+        if (i + 1) % max(1, n_instances // 10) == 0:
+            log(f"  generated {i + 1}/{n_instances} instances, {len(feats)} pairs, "
+                f"{time.perf_counter() - t0:.1f} s")
+    return np.stack(feats), np.stack(targets)
+
+
+def train(net, feats, targets, epochs, batch, lr, seed=0, log=print):
+    rng = np.random.default_rng(seed)
+    n = feats.shape[0]
+    steps = epochs * (n // batch)
+    history = []
+    t0 = time.perf_counter()
+    step = 0
+    for ep in range(epochs):
+        order = rng.permutation(n)
+        losses = []
+        for b in range(n // batch):
+            idx = np.sort(order[b * batch:(b + 1) * batch])
+            # cosine decay from lr to lr/20
+            lr_t = lr * (0.05 + 0.95 * 0.5 * (1 + np.cos(np.pi * step / max(1, steps))))
+            losses.append(net.train_step(feats[idx], targets[idx], lr_t))
+            step += 1
+        history.append(float(np.mean(losses)))
+        log(f"  epoch {ep + 1}/{epochs}: mean loss {history[-1]:.4f} "
+            f"({time.perf_counter() - t0:.1f} s)")
+    return history
+
+
+def validation_loss(net, feats, targets, batch):
+    out = []
+    for b in range(0, feats.shape[0] - batch + 1, batch):
+        loss, _ = net.loss_grad(feats[b:b + batch], targets[b:b + batch])
+        out.append(loss)
+    return float(np.mean(out)) if out else float("nan")
+
+
+def evaluate(net, n_instances, seed0=EVAL_SEED0, n=32, tol=1e-8, restart=30, max_it=4000,
+             log=print):
+    """Plain GMRES(restart) vs FGMRES(restart) with the network, FP64 operator,
+    on held-out instances; returns per-instance records."""
+    quad = ou.Quadrature(ou.LS, 3, sn=8)
+    rows = []
+    for i in range(n_instances):
+        spec = instance(seed0 + i, n=n, quad=quad)
+        op = spec.operator(ou.FP64)
+        phi, a = op.solve(solver=ou.GMRES, tol=tol, max_it=max_it, restart=restart,
+                          raise_on_fail=False)
+        phl, b = op.solve_learned(net, tol=tol, max_it=max_it, restart=restart,
+                                  raise_on_fail=False)
+        op.close()
+        diff = float(np.linalg.norm(phl[:spec.n_dofs] - phi[:spec.n_dofs]) /
+                     np.linalg.norm(phi[:spec.n_dofs]))
+        rows.append({"seed": seed0 + i, "plain_its": a["iterations"],
+                     "plain_s": a["seconds"], "plain_conv": a["converged"],
+                     "learned_its": b["iterations"], "learned_s": b["seconds"],
+                     "learned_conv": b["converged"], "scalar_flux_diff": diff})
+        if (i + 1) % max(1, n_instances // 10) == 0:
+            pi = np.mean([r["plain_its"] for r in rows])
+            li = np.mean([r["learned_its"] for r in rows])
+            log(f"  evaluated {i + 1}/{n_instances}: mean iterations plain {pi:.1f}, "
+                f"learned {li:.1f}")
+    return rows
+
+
+def save(net, path):
+    """Parameters (.npy, float32) plus a JSON sidecar with grid and mode."""
+    np.save(path, net.get_params())
+    with open(os.path.splitext(path)[0] + ".json", "w") as f:
+        json.dump({"grid": net.grid, "mode": net.mode, "n_params": net.n_params}, f)
+
+
+def load(path, max_batch=1):
+    with open(os.path.splitext(path)[0] + ".json") as f:
+        meta = json.load(f)
+    net = ou.UNet(grid=meta["grid"], max_batch=max_batch)
+    net.mode = meta["mode"]
+    net.set_params(np.load(path, allow_pickle=False))
+    return net
+
+
+def summarize(rows):
+    pi = np.array([r["plain_its"] for r in rows], dtype=float)
+    li = np.array([r["learned_its"] for r in rows], dtype=float)
+    ps = np.array([r["plain_s"] for r in rows])
+    ls = np.array([r["learned_s"] for r in rows])
+    return {"instances": len(rows),
+            "plain_iterations_mean": float(pi.mean()), "learned_iterations_mean": float(li.mean()),
+            "iteration_ratio_mean": float(np.mean(pi / np.maximum(li, 1))),
+            "plain_seconds_total": float(ps.sum()), "learned_seconds_total": float(ls.sum()),
+            "speedup_time": float(ps.sum() / max(ls.sum(), 1e-12)),
+            "plain_converged": int(sum(r["plain_conv"] for r in rows)),
+            "learned_converged": int(sum(r["learned_conv"] for r in rows)),
+            "max_scalar_flux_diff": float(max(r["scalar_flux_diff"] for r in rows))}
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    t = sub.add_parser("train")
+    t.add_argument("--instances", type=int, default=2000)
+    t.add_argument("--val-instances", type=int, default=20)
+    t.add_argument("--epochs", type=int, default=8)
+    t.add_argument("--batch", type=int, default=8)
+    t.add_argument("--lr", type=float, default=1e-3)
+    t.add_argument("--grid", type=int, default=32)
+    t.add_argument("--seed", type=int, default=0)
+    t.add_argument("--out", required=True)
+    t.add_argument("--mode", type=int, default=0)
+    t.add_argument("--eval-instances", type=int, default=0)
+    e = sub.add_parser("evaluate")
+    e.add_argument("--weights", required=True)
+    e.add_argument("--instances", type=int, default=200)
+    e.add_argument("--grid", type=int, default=32)
+    e.add_argument("--out", default=None)
+    a = ap.parse_args(argv)
+
+    def log(msg):
+        print(msg, flush=True)
+
+    if a.cmd == "train":
+        log(f"[learned] generating {a.instances} training instances")
+        F, T = generate(a.instances, TRAIN_SEED0, grid=a.grid, log=log)
+        log(f"[learned] generating {a.val_instances} validation instances")
+        Fv, Tv = generate(a.val_instances, EVAL_SEED0 + 100_000, grid=a.grid, log=log)
+        net = ou.UNet(grid=a.grid, max_batch=a.batch, seed=a.seed)
+        net.mode = a.mode
+        v0 = validation_loss(net, Fv, Tv, a.batch)
+        log(f"[learned] {F.shape[0]} pairs; validation loss before training {v0:.4f}")
+        hist = train(net, F, T, a.epochs, a.batch, a.lr, seed=a.seed, log=log)
+        v1 = validation_loss(net, Fv, Tv, a.batch)
+        log(f"[learned] validation loss after training {v1:.4f}")
+        save(net, a.out)
+        rep = {"pairs": int(F.shape[0]), "epochs": a.epochs, "batch": a.batch, "lr": a.lr,
+               "mode": a.mode, "train_loss": hist, "val_loss_before": v0,
+               "val_loss_after": v1}
+        if a.eval_instances:
+            rows = evaluate(net, a.eval_instances, log=log)
+            rep["evaluation"] = summarize(rows)
+            log(json.dumps(rep["evaluation"]))
+        with open(os.path.splitext(a.out)[0] + "_report.json", "w") as f:
+            json.dump(rep, f, indent=1)
+        log("[learned] done")
+    else:
+        net = load(a.weights)
+        rows = evaluate(net, a.instances, log=log)
+        s = summarize(rows)
+        log(json.dumps(s))
+        if a.out:
+            with open(a.out, "w") as f:
+                json.dump({"summary": s, "rows": rows}, f, indent=1)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1:])

@@ -1,0 +1,262 @@
+"""GPU checks of the learned preconditioner: the BF16 MFMA U-Net forward and
+backward against the FP64 oracle (with the GPU's BF16 rounding points
+emulated), training, the coarse-grid transfer, the training-pair generator,
+and FGMRES / FP64-refinement solves that must land on the plain solution."""
+import numpy as np
+import pytest
+
+import ouranos_amd as ou
+from ouranos_amd import problems as P
+from helpers import oracle_problem, rel_err
+from oracle import unet_oracle as U
+
+pytestmark = pytest.mark.gpu
+
+G_TEST = 8
+
+
+def _features(rng, B, G):
+    f = np.zeros((B, G ** 3, U.N_FEAT), dtype=np.float32)
+    f[..., 0:8] = rng.normal(size=(B, G ** 3, 8))
+    f[..., 8] = rng.uniform(-1, 1, size=(B, G ** 3))
+    f[..., 9] = rng.uniform(-1, 0, size=(B, G ** 3))
+    f[..., U.GAIN] = rng.uniform(1, 20, size=(B, G ** 3))
+    return f
+
+
+@pytest.fixture(scope="module")
+def net():
+    n = ou.UNet(grid=G_TEST, max_batch=4, seed=11)
+    yield n
+    n.close()
+
+
+def test_unet_parameter_layout_and_init(net):
+    lay, hw, hb, n = U.layout()
+    assert net.n_params == n
+    p = net.get_params()
+    W0 = p[lay[0][2]:lay[0][3]].reshape(27, 32, 16)
+    assert np.all(W0[..., U.IN_REAL:] == 0.0) and np.std(W0[..., :U.IN_REAL]) > 0.05
+    assert np.all(p[lay[0][3]:lay[0][3] + 32] == 0.0)
+    q = U.init_params(3).astype(np.float32)
+    net.set_params(q)
+    assert np.array_equal(net.get_params(), q)
+
+
+@pytest.mark.parametrize("B,mode", [(1, 0), (2, 3)])
+def test_unet_forward_matches_oracle(net, B, mode):
+    rng = np.random.default_rng(20 + B)
+    p = U.init_params(7 + B).astype(np.float32)
+    net.set_params(p)
+    net.mode = mode
+    feat = _features(rng, B, G_TEST)
+    pred = net.predict(feat).astype(np.float64)
+    ref = U.forward(p, feat, G_TEST, emulate=True, mode=mode)[0].reshape(pred.shape)
+    err = rel_err(pred, ref)
+    # the network's own part, without the gain and the skip connection
+    g = feat[..., U.GAIN:U.GAIN + 1].astype(np.float64) if mode & 2 else 1.0
+    r = feat[..., 0:8].astype(np.float64) if mode & 1 else 0.0
+    err_net = rel_err(pred / g - r, ref / g - r)
+    print(f"[gpu] U-Net forward B={B} mode={mode}: relative error vs emulated-BF16 oracle "
+          f"{err:.2e} (network part {err_net:.2e})")
+    assert err < 1e-2 and err_net < 1e-2
+    # oddness of the prediction in the residual channel
+    f2 = feat.copy()
+    f2[..., 0:8] *= -1
+    assert np.array_equal(net.predict(f2).astype(np.float64), -pred)
+
+
+def _smooth_target(feat, B):
+    """A smooth target field like the training targets: the residual channels
+    averaged over 2^3 blocks of coarse cells."""
+    r = feat[..., 0:8].reshape(B, G_TEST, G_TEST, G_TEST, 8).astype(np.float64)
+    return (3.0 * U.up2(U.pool2(r))).reshape(B, -1).astype(np.float32)
+
+
+@pytest.mark.parametrize("mode,target_kind", [(0, "smooth"), (3, "smooth"), (0, "noise")])
+def test_unet_gradient_matches_oracle(net, mode, target_kind):
+    """GPU gradients (BF16 activations and activation gradients, FP32
+    accumulation) against the FP64 oracle with the same BF16 rounding points.
+
+    The bar calibrates itself: BF16 rounding moves the exact (all-FP64)
+    gradient of this network by several per cent per block (the eight output
+    channels cancel in the head's backward pass), so each block of the GPU
+    gradient must sit at most half that distance from the emulated oracle and
+    point the same way (cosine > 0.995).  Measured: the GPU sits at 0.2-0.4 of
+    the BF16 spread in every block."""
+    rng = np.random.default_rng(31)
+    B = 2
+    p = U.init_params(13).astype(np.float32)
+    net.set_params(p)
+    net.mode = mode
+    feat = _features(rng, B, G_TEST)
+    if target_kind == "smooth":
+        target = _smooth_target(feat, B)
+    else:
+        target = rng.normal(size=(B, G_TEST ** 3 * 8)).astype(np.float32)
+    loss, grad = net.loss_grad(feat, target)
+    loss_ref, grad_ref = U.loss_and_grad(p, feat, target, G_TEST, emulate=True, mode=mode)
+    _, grad_f64 = U.loss_and_grad(p, feat, target, G_TEST, emulate=False, mode=mode)
+    print(f"[gpu] U-Net loss ({target_kind}, mode {mode}) {loss:.6e} oracle {loss_ref:.6e}")
+    assert abs(loss - loss_ref) < 1e-3 * loss_ref
+    lay, hw, hb, n = U.layout()
+    blocks = [(f"layer {i} d{nm}", lo, hi) for i, (cin, cout, w, b) in enumerate(lay)
+              for nm, lo, hi in (("W", w, b), ("b", b, b + cout))] + [("head", hw, n)]
+    bad = []
+    for name, lo, hi in blocks:
+        g, gr = grad[lo:hi].astype(np.float64), grad_ref[lo:hi]
+        e = rel_err(g, gr)
+        spread = rel_err(gr, grad_f64[lo:hi])
+        cos = float(g @ gr / (np.linalg.norm(g) * np.linalg.norm(gr)))
+        print(f"[gpu]   {name}: vs emulated oracle {e:.2e} (BF16 spread {spread:.2e}), "
+              f"cosine {cos:.6f}")
+        if not (e < max(0.5 * spread, 3e-3) and cos > 0.995):
+            bad.append(name)
+    assert not bad, bad
+
+
+def test_unet_training_reduces_loss(net):
+    rng = np.random.default_rng(41)
+    net.set_params(U.init_params(17).astype(np.float32))
+    net.mode = 0
+    feat = _features(rng, 2, G_TEST)
+    # a smooth target the network can fit: a blurred copy of the residual channels
+    r = feat[..., 0:8].reshape(2, G_TEST, G_TEST, G_TEST, 8)
+    t = U.up2(U.pool2(r)).reshape(2, -1).astype(np.float32)
+    losses = [net.train_step(feat, t, 1e-3) for _ in range(40)]
+    print(f"[gpu] training loss {losses[0]:.4f} -> {losses[-1]:.4f}")
+    assert losses[-1] < 0.5 * losses[0]
+
+
+def _small_thick(n=8, block=2, seed=4):
+    return P.thick_3d(n=n, block=block, seed=seed, quad=ou.Quadrature(ou.LS, 3, sn=4))
+
+
+def _octant_matrix(n, G, p):
+    """W[k][X, i, l]: weight of fine node (cell i, local node l) of one axis in
+    half k of coarse cell X (nodes on the mid-plane count half to each side)."""
+    f = n // G
+    W = np.zeros((2, G, n, p + 1))
+    for i in range(n):
+        for l in range(p + 1):
+            s2 = 2 * ((i % f) * p + l)
+            W[0, i // f, i, l] = 1.0 if s2 < f * p else (0.5 if s2 == f * p else 0.0)
+            W[1, i // f, i, l] = 1.0 if s2 > f * p else (0.5 if s2 == f * p else 0.0)
+    return W / W[0].sum(axis=(1, 2))[0]
+
+
+def _octant_restrict(v0, n, G, p):
+    P1 = p + 1
+    A = v0.reshape(n, n, n, P1, P1, P1)  # [kz][ky][kx][lz][ly][lx]
+    W = _octant_matrix(n, G, p)
+    out = np.zeros((G, G, G, 2, 2, 2))  # [Z][Y][X][oz][oy][ox]
+    for oz in range(2):
+        for oy in range(2):
+            for ox in range(2):
+                out[..., oz, oy, ox] = np.einsum("Zal,Ybm,Xcn,abclmn->ZYX", W[oz], W[oy], W[ox], A)
+    return out.reshape(G ** 3, 8)  # octant index ox + 2 oy + 4 oz
+
+
+@pytest.mark.parametrize("p,n,G", [(1, 8, 4), (2, 8, 4), (1, 8, 8), (2, 4, 4)])
+def test_octant_restriction_matches_numpy(p, n, G):
+    spec = P.headline_3d(n=n, block=2, n_polar=2, n_azim=4, L=0, p=p)
+    op = spec.operator(ou.FP64)
+    v = P.seeded_vector(spec.size, 5)
+    coarse, _ = op.coarse_features(G, v)
+    ref = _octant_restrict(v[:spec.n_dofs], n, G, p)
+    assert rel_err(coarse, ref) < 1e-13
+    # constants restrict to the same constant in every octant
+    c1, _ = op.coarse_features(G, np.full(spec.size, 3.5))
+    assert np.allclose(c1, 3.5, rtol=1e-14)
+
+
+@pytest.mark.parametrize("p,n,G", [(1, 8, 4), (2, 8, 4), (1, 8, 8)])
+def test_prolongation_reproduces_linear_fields(p, n, G):
+    """Corner values of a global linear field L prolong to L at every fine node."""
+    spec = P.headline_3d(n=n, block=2, n_polar=2, n_azim=4, L=1, p=p)
+    op = spec.operator(ou.FP64)
+    a = np.array([0.3, -1.2, 0.7, 2.1])
+    L = lambda x, y, z: a[0] + a[1] * x + a[2] * y + a[3] * z  # noqa: E731
+    Zi, Yi, Xi = np.meshgrid(np.arange(G), np.arange(G), np.arange(G), indexing="ij")
+    corners = np.zeros((G, G, G, 8))  # [Z][Y][X][k]
+    for k in range(8):
+        kx, ky, kz = k & 1, (k >> 1) & 1, k >> 2
+        corners[..., k] = L((Xi + kx) / G, (Yi + ky) / G, (Zi + kz) / G)
+    z0 = P.seeded_vector(spec.size, 3)
+    z = op.coarse_prolong(G, corners.reshape(-1), z0)
+    P1 = p + 1
+    i = np.arange(n)[:, None] + np.arange(P1)[None, :] / p  # cell + node fraction
+    xs = (i / n).reshape(-1)  # node positions, cell-major
+    Zn, Yn, Xn = np.meshgrid(xs, xs, xs, indexing="ij")  # [z-node][y-node][x-node]
+    want = L(Xn, Yn, Zn)
+    want = want.reshape(n, P1, n, P1, n, P1).transpose(0, 2, 4, 1, 3, 5).reshape(-1)
+    assert np.allclose(z[:spec.n_dofs] - z0[:spec.n_dofs], want, rtol=1e-6, atol=1e-6)
+    assert np.array_equal(z[spec.n_dofs:], z0[spec.n_dofs:])  # higher moments untouched
+
+
+def test_material_features():
+    spec = _small_thick()
+    op = spec.operator(ou.FP64)
+    _, mat = op.coarse_features(4)
+    n = spec.n[0]
+    # material features: one material per coarse cell (blocks of 2 = f)
+    st = spec.sigma_t[spec.mat].reshape(n, n, n)[::2, ::2, ::2].reshape(-1)
+    c = (spec.sigma_s[:, 0] / spec.sigma_t)[spec.mat].reshape(n, n, n)[::2, ::2, ::2].reshape(-1)
+    tau = st * 2 * spec.h[0]
+    assert np.allclose(mat[:, 0], np.log(tau) / 4, rtol=1e-5, atol=1e-6)
+    assert np.allclose(mat[:, 1], (np.log(1 - c) + 4.6) / 4.6, rtol=1e-4, atol=1e-5)
+    theta = tau ** 2 / (1 + tau ** 2)
+    assert np.allclose(mat[:, 2], c * theta / (1 - c * theta), rtol=1e-5)
+
+
+def test_learned_pairs_follow_the_error_propagation(oracle):
+    spec = _small_thick()
+    op = spec.operator(ou.FP64)
+    pb = oracle_problem(oracle, spec)
+    e0 = P.seeded_vector(spec.size, 9)
+    r, e = op.learned_pairs(4, n_steps=4, record=[0, 3], e0=e0)
+    assert r.shape == (2, 64, 8)
+    # replay the chain with the oracle operator and restrict with the GPU
+    ek = e0.copy()
+    want = []
+    for k in range(4):
+        aek = pb.apply(ek)
+        if k in (0, 3):
+            want.append((op.coarse_features(4, aek)[0], op.coarse_features(4, ek)[0]))
+        ek = ek - aek
+    for j, (wr, we) in enumerate(want):
+        assert rel_err(r[j], wr) < 1e-10
+        assert rel_err(e[j], we) < 1e-12
+
+
+def test_fgmres_with_silent_network_is_plain_gmres():
+    """A network whose head is zero predicts no correction: the preconditioner
+    is then the identity and FGMRES must retrace GMRES."""
+    spec = _small_thick(n=8)
+    op = spec.operator(ou.FP64)
+    phi, info = op.solve(solver=ou.GMRES, tol=1e-11, max_it=3000)
+    n = ou.UNet(grid=8, max_batch=1, seed=5)
+    p = n.get_params()
+    lay, hw, hb, _ = U.layout()
+    p[hw:] = 0.0
+    n.set_params(p)
+    phi_l, info_l = op.solve_learned(n, tol=1e-11, max_it=3000)
+    phi_p, info_p = op.solve_learned(None, tol=1e-11, max_it=3000)
+    print(f"[gpu] GMRES {info['iterations']} its, FGMRES+silent net {info_l['iterations']}, "
+          f"FGMRES plain {info_p['iterations']}")
+    assert info_l["converged"] and info_l["preconditioner_applications"] >= info_l["iterations"]
+    assert info_l["iterations"] == info["iterations"] == info_p["iterations"]
+    assert rel_err(phi_l, phi) < 1e-12
+    assert rel_err(phi_p, phi) < 1e-12
+    n.close()
+
+
+def test_refinement_reaches_fp64_tolerance_with_fp32_sweeps():
+    spec = _small_thick(n=8)
+    op64, op32 = spec.operator(ou.FP64), spec.operator(ou.FP32)
+    phi, info = op64.solve(solver=ou.GMRES, tol=1e-11, max_it=3000)
+    phi_r, info_r = ou.solve_refined(op64, op32, None, tol=1e-10, max_it=3000, inner_tol=1e-4)
+    print(f"[gpu] refinement: {info_r}")
+    assert info_r["converged"] and info_r["rel_residual"] < 1e-10
+    assert info_r["fp64_applications"] >= 2
+    assert rel_err(phi_r, phi) < 1e-7
