@@ -312,7 +312,7 @@ RefinedResult solve_refined(RadiativeTransfer &rt64, RadiativeTransfer &rt32,
     ++res.outer;
     res.total.rel_residual = rn / bnorm;
     if (verbose)
-      std::printf("refinement cycle %d: FP64 residual %.3e after %d FP32 applications\n",
+      progress("refinement cycle %d: FP64 residual %.3e after %d FP32 applications\n",
                   res.outer, res.total.rel_residual, res.total.iterations);
     if (res.total.rel_residual < tol) {
       res.total.converged = true;

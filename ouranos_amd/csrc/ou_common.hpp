@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstdarg>
 #include <cstdio>
 #include <stdexcept>
 #include <string>
@@ -22,6 +23,17 @@ void set_last_error(const std::string &msg);
 
 [[noreturn]] inline void fail(int code, const std::string &msg) {
   throw Error(code, msg);
+}
+
+// Verbose solver progress: printed and flushed at once, so that a caller
+// reading redirected output sees every line as it happens.
+inline void progress(const char *fmt, ...) __attribute__((format(printf, 1, 2)));
+inline void progress(const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  std::vprintf(fmt, ap);
+  va_end(ap);
+  std::fflush(stdout);
 }
 
 inline void hip_check(hipError_t e, const char *what, const char *file, int line) {

@@ -73,7 +73,7 @@ SolveResult solve_si(RadiativeTransfer &rt, double *d_phi, double tol, int max_i
     vec_copy(n, nw.p, d_phi, s);
     res.iterations = it + 1;
     res.rel_residual = den > 0.0 ? num / den : 0.0;
-    if (verbose) std::printf("SI iteration %d: %.3e\n", it, res.rel_residual);
+    if (verbose) progress("SI iteration %d: %.3e\n", it, res.rel_residual);
     if (res.rel_residual < tol) {
       res.converged = true;
       break;
@@ -109,7 +109,7 @@ SolveResult solve_gmres(Operator &op, const double *d_b, double *d_x, int restar
     vec_lincomb(n, 1.0, d_b, -1.0, V.p, V.p, s);
     const double beta = dt.norm(n, V.p, s);
     res.rel_residual = beta / bnorm;
-    if (verbose) std::printf("GMRES restart at %d: true residual %.3e\n", total, res.rel_residual);
+    if (verbose) progress("GMRES restart at %d: true residual %.3e\n", total, res.rel_residual);
     if (res.rel_residual < tol) {
       res.converged = true;
       break;
@@ -160,7 +160,7 @@ SolveResult solve_gmres(Operator &op, const double *d_b, double *d_x, int restar
       g[j + 1] = -sn[j] * g[j];
       g[j] = cs[j] * g[j];
       res.rel_residual = std::fabs(g[j + 1]) / bnorm;
-      if (verbose) std::printf("  it %d: %.3e\n", total, res.rel_residual);
+      if (verbose) progress("  it %d: %.3e\n", total, res.rel_residual);
       if (res.rel_residual < tol || hn == 0.0) {
         ++j;
         break;
@@ -228,7 +228,7 @@ SolveResult solve_bicgstab(Operator &op, const double *d_b, double *d_x, double 
     vec_axpby(n, omega, sv.p, 1.0, d_x, s);
     vec_lincomb(n, 1.0, sv.p, -omega, t.p, r.p, s);
     res.rel_residual = dt.norm(n, r.p, s) / bnorm;
-    if (verbose) std::printf("BiCGStab it %d: %.3e\n", it, res.rel_residual);
+    if (verbose) progress("BiCGStab it %d: %.3e\n", it, res.rel_residual);
   }
   res.iterations = it;
   res.converged = res.rel_residual < tol;

@@ -125,6 +125,44 @@ def train(net, feats, targets, epochs, batch, lr, seed=0, log=print):
     return history
 
 
+def train_streaming(net, n_instances, seed0, grid, chunk, passes, batch, lr, val=None,
+                    sources=("si", "krylov"), seed=0, log=print):
+    """Online training over n_instances generated chunk by chunk: each chunk's
+    pairs get `passes` shuffled passes, then are dropped (the data set of 2000
+    instances does not fit in host memory at once).  Cosine learning-rate
+    decay over the whole run.  Returns per-chunk mean losses (and validation
+    losses if `val` = (features, targets))."""
+    rng = np.random.default_rng(seed)
+    n_chunks = (n_instances + chunk - 1) // chunk
+    est_steps = None
+    step = 0
+    hist, vhist = [], []
+    t0 = time.perf_counter()
+    for c in range(n_chunks):
+        lo = c * chunk
+        F, T = generate(min(chunk, n_instances - lo), seed0 + lo, grid=grid,
+                        log=lambda m: None, sources=sources)
+        if est_steps is None:
+            est_steps = n_chunks * passes * (F.shape[0] // batch)
+        losses = []
+        for _ in range(passes):
+            order = rng.permutation(F.shape[0])
+            for b in range(F.shape[0] // batch):
+                idx = np.sort(order[b * batch:(b + 1) * batch])
+                lr_t = lr * (0.05 + 0.95 * 0.5 * (1 + np.cos(np.pi * min(1.0, step / est_steps))))
+                losses.append(net.train_step(F[idx], T[idx], lr_t))
+                step += 1
+        hist.append(float(np.mean(losses)))
+        msg = (f"  chunk {c + 1}/{n_chunks}: {F.shape[0]} pairs, mean loss {hist[-1]:.4f}, "
+               f"{step} steps, {time.perf_counter() - t0:.0f} s")
+        if val is not None:
+            vhist.append(validation_loss(net, val[0], val[1], batch))
+            msg += f", validation loss {vhist[-1]:.4f}"
+        log(msg)
+        del F, T
+    return hist, vhist
+
+
 def validation_loss(net, feats, targets, batch):
     out = []
     for b in range(0, feats.shape[0] - batch + 1, batch):
@@ -198,7 +236,8 @@ def main(argv=None):
     t = sub.add_parser("train")
     t.add_argument("--instances", type=int, default=2000)
     t.add_argument("--val-instances", type=int, default=20)
-    t.add_argument("--epochs", type=int, default=8)
+    t.add_argument("--chunk", type=int, default=100)
+    t.add_argument("--passes", type=int, default=3)
     t.add_argument("--batch", type=int, default=8)
     t.add_argument("--lr", type=float, default=1e-3)
     t.add_argument("--grid", type=int, default=32)
@@ -217,21 +256,27 @@ def main(argv=None):
         print(msg, flush=True)
 
     if a.cmd == "train":
-        log(f"[learned] generating {a.instances} training instances")
-        F, T = generate(a.instances, TRAIN_SEED0, grid=a.grid, log=log)
         log(f"[learned] generating {a.val_instances} validation instances")
         Fv, Tv = generate(a.val_instances, EVAL_SEED0 + 100_000, grid=a.grid, log=log)
         net = ou.UNet(grid=a.grid, max_batch=a.batch, seed=a.seed)
         net.mode = a.mode
         v0 = validation_loss(net, Fv, Tv, a.batch)
-        log(f"[learned] {F.shape[0]} pairs; validation loss before training {v0:.4f}")
-        hist = train(net, F, T, a.epochs, a.batch, a.lr, seed=a.seed, log=log)
-        v1 = validation_loss(net, Fv, Tv, a.batch)
-        log(f"[learned] validation loss after training {v1:.4f}")
+        log(f"[learned] validation loss before training {v0:.4f}; training on {a.instances} "
+            f"instances in chunks of {a.chunk}, {a.passes} passes each")
+        t0 = time.perf_counter()
+        hist, vhist = train_streaming(net, a.instances, TRAIN_SEED0, a.grid, a.chunk, a.passes,
+                                      a.batch, a.lr, val=(Fv, Tv), seed=a.seed, log=log)
+        secs = time.perf_counter() - t0
+        v1 = vhist[-1]
+        log(f"[learned] validation loss after training {v1:.4f} ({secs:.0f} s)")
         save(net, a.out)
-        rep = {"pairs": int(F.shape[0]), "epochs": a.epochs, "batch": a.batch, "lr": a.lr,
-               "mode": a.mode, "train_loss": hist, "val_loss_before": v0,
-               "val_loss_after": v1}
+        rep = {"instances": a.instances, "chunk": a.chunk, "passes": a.passes,
+               "batch": a.batch, "lr": a.lr, "mode": a.mode, "seconds": secs,
+               "train_loss": hist, "val_loss": vhist, "val_loss_before": v0,
+               "val_loss_after": v1,
+               "sources": "SI error-propagation chains (RHS and random field, steps "
+                          f"{list(RECORD)}) + plain GMRES(30) Arnoldi pairs (every "
+                          f"{KRYLOV_STRIDE}th of {KRYLOV_STEPS})"}
         if a.eval_instances:
             rows = evaluate(net, a.eval_instances, log=log)
             rep["evaluation"] = summarize(rows)
