@@ -303,17 +303,24 @@ RefinedResult solve_refined(RadiativeTransfer &rt64, RadiativeTransfer &rt32,
     return res;
   }
   double fp64_secs = 0.0;
-  while (true) {
+  // A zero initial guess has residual b: the first FP64 application is skipped.
+  bool zero_guess = norm(d_phi) == 0.0;
+  for (int cycle = 1;; ++cycle) {
     const auto a0 = std::chrono::steady_clock::now();
-    rt64.Apply(d_phi, r, s);
-    vec_lincomb(n, 1.0, b, -1.0, r, r, s);
+    if (zero_guess) {
+      vec_copy(n, b, r, s);
+      zero_guess = false;
+    } else {
+      rt64.Apply(d_phi, r, s);
+      vec_lincomb(n, 1.0, b, -1.0, r, r, s);
+      ++res.outer;
+    }
     const double rn = norm(r);
     fp64_secs += std::chrono::duration<double>(std::chrono::steady_clock::now() - a0).count();
-    ++res.outer;
     res.total.rel_residual = rn / bnorm;
     if (verbose)
-      progress("refinement cycle %d: FP64 residual %.3e after %d FP32 applications\n",
-                  res.outer, res.total.rel_residual, res.total.iterations);
+      progress("refinement cycle %d: FP64 residual %.3e after %d FP32 applications\n", cycle,
+               res.total.rel_residual, res.total.iterations);
     if (res.total.rel_residual < tol) {
       res.total.converged = true;
       break;

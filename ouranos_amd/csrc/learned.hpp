@@ -62,7 +62,8 @@ class LearnedPreconditioner : public Preconditioner {
 
 struct RefinedResult {
   SolveResult total;  // iterations = FP32 (inner) operator applications
-  int outer = 0;      // refinement cycles (= FP64 operator applications)
+  int outer = 0;      // FP64 residual evaluations (operator applications; a zero
+                      // initial guess needs none for the first cycle)
   double seconds_fp64 = 0.0;
 };
 

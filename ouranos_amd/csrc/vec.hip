@@ -1,4 +1,12 @@
 // FP64 vector kernels for the Krylov drivers; see vec.hpp.
+//
+// All kernels stream HBM: each thread walks a grid-strided range in 16-byte
+// (double2) steps when every operand is 16-byte aligned and the leading
+// dimension is even, else in 8-byte steps.  Dot products keep one FP64
+// accumulator per vector in registers, reduce across the wave with shuffles,
+// then across the block's four waves in LDS, and leave one partial per
+// block; a second kernel sums the partials in a fixed order, so results are
+// deterministic for a given n.
 #include "ou_common.hpp"
 #include "vec.hpp"
 
@@ -7,7 +15,8 @@ namespace ou {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kMaxBlocks = 1024;
+constexpr int kWaves = kBlock / 64;
+constexpr int kMaxBlocks = 2048;
 constexpr int kMaxDots = 32;
 
 inline int blocks_for(long n) {
@@ -15,90 +24,152 @@ inline int blocks_for(long n) {
   return int(b < kMaxBlocks ? (b < 1 ? 1 : b) : kMaxBlocks);
 }
 
-// Partial sums of m dot products per block; each thread walks a
-// grid-strided range, then the block reduces in LDS.
-template <int M>
+inline bool aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
+
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+// Partial sums of m dot products <V_i, w> per block.
+template <int M, int VEC>
 __global__ __launch_bounds__(kBlock) void k_multi_dot(long n, int m, const double *V,
                                                       long ldv, const double *w,
                                                       double *partial) {
-  __shared__ double red[kBlock];
+  __shared__ double red[kWaves][M];
   double acc[M];
 #pragma unroll
   for (int i = 0; i < M; ++i) acc[i] = 0.0;
-  for (long k = long(blockIdx.x) * kBlock + threadIdx.x; k < n; k += long(gridDim.x) * kBlock) {
-    const double wk = w[k];
+  const long stride = long(gridDim.x) * kBlock;
+  if (VEC == 2) {
+    const long n2 = n / 2;
+    const double2 *w2 = reinterpret_cast<const double2 *>(w);
+    for (long k = long(blockIdx.x) * kBlock + threadIdx.x; k < n2; k += stride) {
+      const double2 wk = w2[k];
 #pragma unroll
-    for (int i = 0; i < M; ++i)
-      if (i < m) acc[i] += V[i * ldv + k] * wk;
-  }
-  for (int i = 0; i < m; ++i) {
-    red[threadIdx.x] = acc[i < M ? i : 0];
-    __syncthreads();
-    for (int s = kBlock / 2; s > 0; s >>= 1) {
-      if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-      __syncthreads();
+      for (int i = 0; i < M; ++i)
+        if (i < m) {
+          const double2 v = reinterpret_cast<const double2 *>(V + i * ldv)[k];
+          acc[i] += v.x * wk.x + v.y * wk.y;
+        }
     }
-    if (threadIdx.x == 0) partial[long(blockIdx.x) * m + i] = red[0];
-    __syncthreads();
+    if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0)
+      for (int i = 0; i < M; ++i)
+        if (i < m) acc[i] += V[i * ldv + n - 1] * w[n - 1];
+  } else {
+    for (long k = long(blockIdx.x) * kBlock + threadIdx.x; k < n; k += stride) {
+      const double wk = w[k];
+#pragma unroll
+      for (int i = 0; i < M; ++i)
+        if (i < m) acc[i] += V[i * ldv + k] * wk;
+    }
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int i = 0; i < M; ++i)
+    if (i < m) {
+      const double s = wave_sum(acc[i]);
+      if (lane == 0) red[wave][i] = s;
+    }
+  __syncthreads();
+  if (threadIdx.x < m) {
+    double s = 0.0;
+    for (int q = 0; q < kWaves; ++q) s += red[q][threadIdx.x];
+    partial[long(blockIdx.x) * m + threadIdx.x] = s;
   }
 }
 
 __global__ __launch_bounds__(kBlock) void k_reduce_partials(int nb, int m,
                                                             const double *partial,
                                                             double *out) {
-  __shared__ double red[kBlock];
+  __shared__ double red[kWaves];
   for (int i = 0; i < m; ++i) {
     double s = 0.0;
     for (int b = threadIdx.x; b < nb; b += kBlock) s += partial[long(b) * m + i];
-    red[threadIdx.x] = s;
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
     __syncthreads();
-    for (int st = kBlock / 2; st > 0; st >>= 1) {
-      if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
-      __syncthreads();
+    if (threadIdx.x == 0) {
+      double t = 0.0;
+      for (int q = 0; q < kWaves; ++q) t += red[q];
+      out[i] = t;
     }
-    if (threadIdx.x == 0) out[i] = red[0];
     __syncthreads();
   }
 }
 
+// w -= alpha sum_i h[i] V_i
+template <int VEC>
 __global__ __launch_bounds__(kBlock) void k_multi_axpy(long n, int m, const double *V,
                                                        long ldv, const double *h,
                                                        double alpha, double *w) {
   __shared__ double hs[64];
-  if (threadIdx.x < m) hs[threadIdx.x] = h[threadIdx.x];
+  if (threadIdx.x < m) hs[threadIdx.x] = alpha * h[threadIdx.x];
   __syncthreads();
-  for (long k = long(blockIdx.x) * kBlock + threadIdx.x; k < n; k += long(gridDim.x) * kBlock) {
-    double s = 0.0;
-    for (int i = 0; i < m; ++i) s += hs[i] * V[i * ldv + k];
-    w[k] -= alpha * s;
+  const long stride = long(gridDim.x) * kBlock;
+  if (VEC == 2) {
+    const long n2 = n / 2;
+    double2 *w2 = reinterpret_cast<double2 *>(w);
+    for (long k = long(blockIdx.x) * kBlock + threadIdx.x; k < n2; k += stride) {
+      double sx = 0.0, sy = 0.0;
+      for (int i = 0; i < m; ++i) {
+        const double2 v = reinterpret_cast<const double2 *>(V + i * ldv)[k];
+        sx += hs[i] * v.x;
+        sy += hs[i] * v.y;
+      }
+      double2 wk = w2[k];
+      wk.x -= sx;
+      wk.y -= sy;
+      w2[k] = wk;
+    }
+    if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) {
+      double s = 0.0;
+      for (int i = 0; i < m; ++i) s += hs[i] * V[i * ldv + n - 1];
+      w[n - 1] -= s;
+    }
+  } else {
+    for (long k = long(blockIdx.x) * kBlock + threadIdx.x; k < n; k += stride) {
+      double s = 0.0;
+      for (int i = 0; i < m; ++i) s += hs[i] * V[i * ldv + k];
+      w[k] -= s;
+    }
   }
 }
 
-__global__ __launch_bounds__(kBlock) void k_axpby(long n, double a, const double *x,
-                                                  double b, double *y) {
-  for (long k = long(blockIdx.x) * kBlock + threadIdx.x; k < n; k += long(gridDim.x) * kBlock)
-    y[k] = a * x[k] + b * y[k];
+// Element-wise kernels: z = a x + b y (x, y may be null when unused), in
+// double2 steps when aligned (VEC 2), with a scalar tail for odd n.
+// kind: 0 axpby (y = a x + b y), 1 lincomb (z = a x + b y), 2 scale (x *= a),
+// 3 fill (x = a), 4 copy (y = x).
+__device__ __forceinline__ double elem(int kind, double a, double b, double xv, double yv) {
+  switch (kind) {
+    case 0:
+    case 1: return a * xv + b * yv;
+    case 2: return a * xv;
+    case 3: return a;
+    default: return xv;
+  }
 }
 
-__global__ __launch_bounds__(kBlock) void k_lincomb(long n, double a, const double *x,
-                                                    double b, const double *y, double *z) {
-  for (long k = long(blockIdx.x) * kBlock + threadIdx.x; k < n; k += long(gridDim.x) * kBlock)
-    z[k] = a * x[k] + b * y[k];
-}
-
-__global__ __launch_bounds__(kBlock) void k_scale(long n, double a, double *x) {
-  for (long k = long(blockIdx.x) * kBlock + threadIdx.x; k < n; k += long(gridDim.x) * kBlock)
-    x[k] *= a;
-}
-
-__global__ __launch_bounds__(kBlock) void k_fill(long n, double v, double *x) {
-  for (long k = long(blockIdx.x) * kBlock + threadIdx.x; k < n; k += long(gridDim.x) * kBlock)
-    x[k] = v;
-}
-
-__global__ __launch_bounds__(kBlock) void k_copy(long n, const double *x, double *y) {
-  for (long k = long(blockIdx.x) * kBlock + threadIdx.x; k < n; k += long(gridDim.x) * kBlock)
-    y[k] = x[k];
+template <int KIND, int VEC>
+__global__ __launch_bounds__(kBlock) void k_elementwise(long n, double a, const double *x,
+                                                        double b, const double *y, double *z) {
+  const long stride = long(gridDim.x) * kBlock;
+  const bool rx = KIND != 3, ry = KIND <= 1;
+  if (VEC == 2) {
+    const double2 *x2 = reinterpret_cast<const double2 *>(x);
+    const double2 *y2 = reinterpret_cast<const double2 *>(y);
+    double2 *z2 = reinterpret_cast<double2 *>(z);
+    for (long k = long(blockIdx.x) * kBlock + threadIdx.x; k < n / 2; k += stride) {
+      const double2 xv = rx ? x2[k] : double2{0.0, 0.0};
+      const double2 yv = ry ? y2[k] : double2{0.0, 0.0};
+      z2[k] = double2{elem(KIND, a, b, xv.x, yv.x), elem(KIND, a, b, xv.y, yv.y)};
+    }
+    if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0)
+      z[n - 1] = elem(KIND, a, b, rx ? x[n - 1] : 0.0, ry ? y[n - 1] : 0.0);
+  } else {
+    for (long k = long(blockIdx.x) * kBlock + threadIdx.x; k < n; k += stride)
+      z[k] = elem(KIND, a, b, rx ? x[k] : 0.0, ry ? y[k] : 0.0);
+  }
 }
 
 __global__ __launch_bounds__(kBlock) void k_zero_words(long n, unsigned *p) {
@@ -113,6 +184,17 @@ __global__ __launch_bounds__(kBlock) void k_check_finite(long n, const double *x
   if (bad) atomicOr(flag, 1);
 }
 
+template <int M>
+void launch_dot(bool v2, int nb, hipStream_t s, long n, int m, const double *V, long ldv,
+                const double *w, double *scratch) {
+  if (v2)
+    hipLaunchKernelGGL((k_multi_dot<M, 2>), dim3(nb), dim3(kBlock), 0, s, n, m, V, ldv, w,
+                       scratch);
+  else
+    hipLaunchKernelGGL((k_multi_dot<M, 1>), dim3(nb), dim3(kBlock), 0, s, n, m, V, ldv, w,
+                       scratch);
+}
+
 }  // namespace
 
 long reduce_scratch_doubles(int m) { return long(kMaxBlocks) * (m < 1 ? 1 : m); }
@@ -121,14 +203,16 @@ void vec_multi_dot(long n, int m, const double *V, long ldv, const double *w,
                    double *scratch, double *out, hipStream_t s) {
   if (m < 1) return;
   if (m > kMaxDots) fail(OU_ERR_ARG, "too many simultaneous dot products");
-  const int nb = blocks_for(n);
+  const bool v2 = aligned16(V) && aligned16(w) && (m == 1 || ldv % 2 == 0);
+  const int nb = blocks_for(v2 ? (n + 1) / 2 : n);
   if (m == 1)
-    hipLaunchKernelGGL(k_multi_dot<1>, dim3(nb), dim3(kBlock), 0, s, n, m, V, ldv, w, scratch);
+    launch_dot<1>(v2, nb, s, n, m, V, ldv, w, scratch);
   else if (m <= 8)
-    hipLaunchKernelGGL(k_multi_dot<8>, dim3(nb), dim3(kBlock), 0, s, n, m, V, ldv, w, scratch);
+    launch_dot<8>(v2, nb, s, n, m, V, ldv, w, scratch);
+  else if (m <= 16)
+    launch_dot<16>(v2, nb, s, n, m, V, ldv, w, scratch);
   else
-    hipLaunchKernelGGL(k_multi_dot<kMaxDots>, dim3(nb), dim3(kBlock), 0, s, n, m, V, ldv, w,
-                       scratch);
+    launch_dot<kMaxDots>(v2, nb, s, n, m, V, ldv, w, scratch);
   hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, s, nb, m, scratch, out);
   OU_HIP(hipGetLastError());
 }
@@ -137,35 +221,50 @@ void vec_multi_axpy(long n, int m, const double *V, long ldv, const double *h, d
                     double *w, hipStream_t s) {
   if (m < 1) return;
   if (m > 64) fail(OU_ERR_ARG, "multi_axpy supports at most 64 vectors");
-  hipLaunchKernelGGL(k_multi_axpy, dim3(blocks_for(n)), dim3(kBlock), 0, s, n, m, V, ldv, h,
-                     alpha, w);
+  const bool v2 = aligned16(V) && aligned16(w) && (m == 1 || ldv % 2 == 0);
+  const int nb = blocks_for(v2 ? (n + 1) / 2 : n);
+  if (v2)
+    hipLaunchKernelGGL(k_multi_axpy<2>, dim3(nb), dim3(kBlock), 0, s, n, m, V, ldv, h, alpha, w);
+  else
+    hipLaunchKernelGGL(k_multi_axpy<1>, dim3(nb), dim3(kBlock), 0, s, n, m, V, ldv, h, alpha, w);
   OU_HIP(hipGetLastError());
 }
 
-void vec_axpby(long n, double a, const double *x, double b, double *y, hipStream_t s) {
-  hipLaunchKernelGGL(k_axpby, dim3(blocks_for(n)), dim3(kBlock), 0, s, n, a, x, b, y);
+namespace {
+
+template <int KIND>
+void elementwise(long n, double a, const double *x, double b, const double *y, double *z,
+                 hipStream_t s) {
+  const bool v2 = (!x || aligned16(x)) && (!y || aligned16(y)) && aligned16(z);
+  const int nb = blocks_for(v2 ? (n + 1) / 2 : n);
+  if (v2)
+    hipLaunchKernelGGL((k_elementwise<KIND, 2>), dim3(nb), dim3(kBlock), 0, s, n, a, x, b, y, z);
+  else
+    hipLaunchKernelGGL((k_elementwise<KIND, 1>), dim3(nb), dim3(kBlock), 0, s, n, a, x, b, y, z);
   OU_HIP(hipGetLastError());
+}
+
+}  // namespace
+
+void vec_axpby(long n, double a, const double *x, double b, double *y, hipStream_t s) {
+  elementwise<0>(n, a, x, b, y, y, s);
 }
 
 void vec_lincomb(long n, double a, const double *x, double b, const double *y, double *z,
                  hipStream_t s) {
-  hipLaunchKernelGGL(k_lincomb, dim3(blocks_for(n)), dim3(kBlock), 0, s, n, a, x, b, y, z);
-  OU_HIP(hipGetLastError());
+  elementwise<1>(n, a, x, b, y, z, s);
 }
 
 void vec_scale(long n, double a, double *x, hipStream_t s) {
-  hipLaunchKernelGGL(k_scale, dim3(blocks_for(n)), dim3(kBlock), 0, s, n, a, x);
-  OU_HIP(hipGetLastError());
+  elementwise<2>(n, a, x, 0.0, nullptr, x, s);
 }
 
 void vec_copy(long n, const double *x, double *y, hipStream_t s) {
-  hipLaunchKernelGGL(k_copy, dim3(blocks_for(n)), dim3(kBlock), 0, s, n, x, y);
-  OU_HIP(hipGetLastError());
+  elementwise<4>(n, 0.0, x, 0.0, nullptr, y, s);
 }
 
 void vec_fill(long n, double v, double *x, hipStream_t s) {
-  hipLaunchKernelGGL(k_fill, dim3(blocks_for(n)), dim3(kBlock), 0, s, n, v, x);
-  OU_HIP(hipGetLastError());
+  elementwise<3>(n, v, nullptr, 0.0, nullptr, x, s);
 }
 
 void vec_zero_bytes(void *p, size_t bytes, hipStream_t s) {

@@ -34,6 +34,29 @@ RECORD = (0, 2, 6, 14)    # recorded steps k of each chain
 N_STEPS = 15
 
 
+class Family:
+    """A seeded family of training/evaluation instances.
+    thick:    32^3 DG Q1, LS-S8, 4^3-cell blocks, sigma_t log-U[1,1000],
+              c U[0.99,0.9999] (the learned-preconditioner data set), FP64 sweeps;
+    headline: 64^3 DG Q2, product GLC 12x24, L=1, 8^3-cell blocks with the
+              headline materials (sigma_t log-U[1,500], c U[0.99,0.9999]); on
+              the 32^3 coarse grid its coarse cells and blocks have the optical
+              sizes of the 128^3 headline problem's; FP32 sweeps."""
+
+    def __init__(self, name):
+        if name not in ("thick", "headline"):
+            raise ValueError(f"unknown family {name}")
+        self.name = name
+        self.precision = ou.FP64 if name == "thick" else ou.FP32
+        self.quad = (ou.Quadrature(ou.LS, 3, sn=8) if name == "thick" else
+                     ou.Quadrature(ou.PRODUCT_GLC, 3, n_polar=12, n_azim=24, L_max=1))
+
+    def spec(self, seed):
+        if self.name == "thick":
+            return P.thick_3d(n=32, block=4, seed=seed, quad=self.quad)
+        return P.headline_3d(n=64, block=8, seed=seed, quad=self.quad)
+
+
 def instance(seed, n=32, quad=None):
     return P.thick_3d(n=n, block=4, seed=seed, quad=quad)
 
@@ -86,13 +109,14 @@ def pairs_of(op, spec, grid, seed, sources=("si", "krylov")):
     return feats, targets
 
 
-def generate(n_instances, seed0, grid=32, n=32, log=print, sources=("si", "krylov")):
-    quad = ou.Quadrature(ou.LS, 3, sn=8)
+def generate(n_instances, seed0, grid=32, n=32, log=print, sources=("si", "krylov"),
+             family=None):
+    fam = family or Family("thick")
     feats, targets = [], []
     t0 = time.perf_counter()
     for i in range(n_instances):
-        spec = instance(seed0 + i, n=n, quad=quad)
-        op = spec.operator(ou.FP64)
+        spec = fam.spec(seed0 + i) if family else instance(seed0 + i, n=n, quad=fam.quad)
+        op = spec.operator(fam.precision)
         f, t = pairs_of(op, spec, grid, seed0 + i, sources)
         op.close()
         feats += f
@@ -126,7 +150,7 @@ def train(net, feats, targets, epochs, batch, lr, seed=0, log=print):
 
 
 def train_streaming(net, n_instances, seed0, grid, chunk, passes, batch, lr, val=None,
-                    sources=("si", "krylov"), seed=0, log=print):
+                    sources=("si", "krylov"), seed=0, log=print, family=None):
     """Online training over n_instances generated chunk by chunk: each chunk's
     pairs get `passes` shuffled passes, then are dropped (the data set of 2000
     instances does not fit in host memory at once).  Cosine learning-rate
@@ -141,7 +165,7 @@ def train_streaming(net, n_instances, seed0, grid, chunk, passes, batch, lr, val
     for c in range(n_chunks):
         lo = c * chunk
         F, T = generate(min(chunk, n_instances - lo), seed0 + lo, grid=grid,
-                        log=lambda m: None, sources=sources)
+                        log=lambda m: None, sources=sources, family=family)
         if est_steps is None:
             est_steps = n_chunks * passes * (F.shape[0] // batch)
         losses = []
@@ -231,6 +255,7 @@ def summarize(rows):
 
 
 def main(argv=None):
+    global KRYLOV_STEPS, KRYLOV_STRIDE
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
     t = sub.add_parser("train")
@@ -244,6 +269,13 @@ def main(argv=None):
     t.add_argument("--seed", type=int, default=0)
     t.add_argument("--out", required=True)
     t.add_argument("--mode", type=int, default=0)
+    t.add_argument("--family", default="thick", choices=["thick", "headline"])
+    t.add_argument("--sources", default="si,krylov")
+    t.add_argument("--init", default=None, help="start from these weights")
+    t.add_argument("--seed-offset", type=int, default=0,
+                   help="first training instance seed = TRAIN_SEED0 + offset")
+    t.add_argument("--krylov-steps", type=int, default=KRYLOV_STEPS)
+    t.add_argument("--krylov-stride", type=int, default=KRYLOV_STRIDE)
     t.add_argument("--eval-instances", type=int, default=0)
     e = sub.add_parser("evaluate")
     e.add_argument("--weights", required=True)
@@ -256,16 +288,27 @@ def main(argv=None):
         print(msg, flush=True)
 
     if a.cmd == "train":
-        log(f"[learned] generating {a.val_instances} validation instances")
-        Fv, Tv = generate(a.val_instances, EVAL_SEED0 + 100_000, grid=a.grid, log=log)
+        KRYLOV_STEPS, KRYLOV_STRIDE = a.krylov_steps, a.krylov_stride
+        fam = Family(a.family)
+        sources = tuple(a.sources.split(","))
+        log(f"[learned] generating {a.val_instances} validation instances ({a.family})")
+        Fv, Tv = generate(a.val_instances, EVAL_SEED0 + 100_000, grid=a.grid, log=log,
+                          sources=sources, family=fam)
         net = ou.UNet(grid=a.grid, max_batch=a.batch, seed=a.seed)
         net.mode = a.mode
+        if a.init:
+            init = load(a.init)
+            net.mode = init.mode
+            net.set_params(init.get_params())
+            init.close()
         v0 = validation_loss(net, Fv, Tv, a.batch)
         log(f"[learned] validation loss before training {v0:.4f}; training on {a.instances} "
             f"instances in chunks of {a.chunk}, {a.passes} passes each")
         t0 = time.perf_counter()
-        hist, vhist = train_streaming(net, a.instances, TRAIN_SEED0, a.grid, a.chunk, a.passes,
-                                      a.batch, a.lr, val=(Fv, Tv), seed=a.seed, log=log)
+        hist, vhist = train_streaming(net, a.instances, TRAIN_SEED0 + a.seed_offset, a.grid,
+                                      a.chunk, a.passes,
+                                      a.batch, a.lr, val=(Fv, Tv), seed=a.seed, log=log,
+                                      sources=sources, family=fam)
         secs = time.perf_counter() - t0
         v1 = vhist[-1]
         log(f"[learned] validation loss after training {v1:.4f} ({secs:.0f} s)")
@@ -273,10 +316,13 @@ def main(argv=None):
         rep = {"instances": a.instances, "chunk": a.chunk, "passes": a.passes,
                "batch": a.batch, "lr": a.lr, "mode": a.mode, "seconds": secs,
                "train_loss": hist, "val_loss": vhist, "val_loss_before": v0,
-               "val_loss_after": v1,
-               "sources": "SI error-propagation chains (RHS and random field, steps "
-                          f"{list(RECORD)}) + plain GMRES(30) Arnoldi pairs (every "
-                          f"{KRYLOV_STRIDE}th of {KRYLOV_STEPS})"}
+               "val_loss_after": v1, "family": a.family, "family_doc": Family.__doc__,
+               "sources": {"si": "SI error-propagation chains (RHS and random field, steps "
+                                 f"{list(RECORD)})",
+                           "krylov": f"plain GMRES(30) Arnoldi pairs (every {KRYLOV_STRIDE}th "
+                                     f"of {KRYLOV_STEPS})"},
+               "sources_used": list(sources),
+               "train_command": "python -m ouranos_amd.learned " + " ".join(argv or sys.argv[1:])}
         if a.eval_instances:
             rows = evaluate(net, a.eval_instances, log=log)
             rep["evaluation"] = summarize(rows)

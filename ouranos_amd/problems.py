@@ -178,8 +178,9 @@ def thick_3d(n=32, block=4, seed=4, sn=8, ws=4.0 * np.pi, quad=None):
 
 
 def headline_3d(n=128, block=16, seed=5, n_polar=12, n_azim=24, L=1, p=2,
-                ws=4.0 * np.pi):
-    quad = Quadrature(PRODUCT_GLC, 3, n_polar=n_polar, n_azim=n_azim, L_max=L, weight_sum=ws)
+                ws=4.0 * np.pi, quad=None):
+    quad = quad or Quadrature(PRODUCT_GLC, 3, n_polar=n_polar, n_azim=n_azim, L_max=L,
+                              weight_sum=ws)
     mat, nb = _block_map((n, n, n), (block,) * 3)
     rng = SplitMix64(seed)
     st, ss, qb = np.zeros(nb), np.zeros((nb, L + 1)), np.zeros(nb)
