@@ -274,6 +274,72 @@ int ou_solve_refined(ou_plan *plan64, ou_plan *plan32, ou_unet *net,
                      ou_solve_info *info);
 
 /* ------------------------------------------------------------------------ */
+/* Sweep workgroup shape and the batched multi-instance launch.             */
+/*                                                                          */
+/* The reference sweeps one problem per process (RadiativeTransfer::Apply,  */
+/* src/RadiativeTransfer.cc:781-809, called once per Krylov step); these    */
+/* calls have no counterpart there.  They replace a host loop of ou_apply   */
+/* calls over many independent plans: the members' wavefront levels are     */
+/* merged (one sweep launch per level of the deepest member), the           */
+/* element-wise stages are one launch each for the whole batch, and the     */
+/* sequence is captured once as a HIP graph and replayed.                   */
+/* ------------------------------------------------------------------------ */
+
+/* Waves per sweep workgroup (directions in flight per chunk of 64 cells) and
+ * direction groups per octant (workgroups per chunk, each with a range of the
+ * octant's directions and its own FP64 partial moments, summed in the
+ * epilogue).  Shapes change the order of FP64 additions only. */
+int ou_set_sweep_shape(ou_plan *plan, int waves, int dir_groups);
+int ou_sweep_shape(const ou_plan *plan, int *waves, int *dir_groups);
+
+typedef struct ou_batch ou_batch;
+
+typedef struct {
+  int waves;
+  int dir_groups;
+  double ms; /* mean event time of one batched application */
+} ou_tune_entry;
+
+/* A batch of n_plans distinct plans sharing dimension, element order, moment
+ * count and precision (meshes, materials, quadratures and sources may all
+ * differ).  waves / dir_groups <= 0: the autotuner picks the shape, from its
+ * cache (OURANOS_TUNE_CACHE names a file that persists it) or by timing every
+ * supported shape on this batch.  The batch sets every member's sweep shape;
+ * members keep their own state (scattering source, traces, partials), so
+ * batched and single calls on one plan must be ordered by the caller, as
+ * must batched calls on one batch issued to different streams.  The plans
+ * must outlive the batch. */
+int ou_batch_create(ou_plan *const *plans, int n_plans, int waves, int dir_groups,
+                    ou_batch **batch);
+void ou_batch_destroy(ou_batch *batch);
+int ou_batch_info(const ou_batch *batch, int *waves, int *dir_groups, int *n_levels,
+                  long *n_chunks);
+int ou_batch_set_shape(ou_batch *batch, int waves, int dir_groups);
+/* use_graph = 0 launches the same kernels eagerly (for comparison). */
+int ou_batch_set_graph(ou_batch *batch, int use_graph);
+
+/* Apply of every member: d_y[i] = d_x[i] - D L^-1 M S d_x[i] (device vectors;
+ * d_x / d_y are host arrays of n_plans device pointers; all d_y distinct and
+ * distinct from the d_x).  stream NULL = the batch's stream.  No host
+ * synchronisation beyond the upload of the pointer table. */
+int ou_batch_apply_device(ou_batch *batch, const double *const *d_x, double *const *d_y,
+                          void *stream);
+/* ou_sweep_device of every member: d_y[i] += D2M L^-1 (M2D S_i [+ M q_i/ws +
+ * inflow if with_source]). */
+int ou_batch_sweep_device(ou_batch *batch, double *const *d_y, int with_source,
+                          void *stream);
+
+/* Re-runs the autotuner (reps timed applications per shape), keeps the
+ * fastest shape and writes up to max_entries timings. */
+int ou_batch_tune(ou_batch *batch, int reps, ou_tune_entry *entries, int max_entries,
+                  int *n_entries);
+
+/* `warmup` untimed then `steps` timed batched applications on the batch's
+ * stream; *ms = mean event time of one. */
+int ou_batch_time_apply(ou_batch *batch, const double *const *d_x, double *const *d_y,
+                        int warmup, int steps, double *ms);
+
+/* ------------------------------------------------------------------------ */
 /* Device memory and timing plumbing (for callers without a HIP runtime of  */
 /* their own, e.g. ctypes).                                                 */
 /* ------------------------------------------------------------------------ */

@@ -22,7 +22,7 @@ __all__ = [
     "Quadrature", "RadiativeTransfer", "fe_matrices", "OuError", "lib", "LIB_PATH",
     "FP64", "FP32", "LS", "GLC", "PRODUCT_GLC", "SI", "GMRES", "BICGSTAB", "FGMRES",
     "BC_VACUUM", "BC_ISOTROPIC", "BC_MOST_NORMAL", "DeviceBuffer", "device_name", "UNet",
-    "solve_refined",
+    "solve_refined", "Batch",
 ]
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libouranos_amd.so")
@@ -75,6 +75,10 @@ class SolveInfo(ctypes.Structure):
                 "seconds": self.seconds, "converged": bool(self.converged),
                 "fp64_applications": self.fp64_applications,
                 "preconditioner_applications": self.preconditioner_applications}
+
+
+class TuneEntry(ctypes.Structure):
+    _fields_ = [("waves", ctypes.c_int), ("dir_groups", ctypes.c_int), ("ms", ctypes.c_double)]
 
 
 class Timing(ctypes.Structure):
@@ -136,6 +140,22 @@ def lib():
                                        ctypes.POINTER(SolveInfo)]
         L.ou_solve_refined.argtypes = [_vp, _vp, _vp, ctypes.POINTER(SolverOpts),
                                        ctypes.c_double, _dp, ctypes.POINTER(SolveInfo)]
+        L.ou_set_sweep_shape.argtypes = [_vp, ctypes.c_int, ctypes.c_int]
+        L.ou_sweep_shape.argtypes = [_vp, _ip, _ip]
+        L.ou_batch_create.argtypes = [ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int,
+                                      ctypes.c_int, ctypes.POINTER(_vp)]
+        L.ou_batch_destroy.argtypes = [_vp]
+        L.ou_batch_destroy.restype = None
+        L.ou_batch_info.argtypes = [_vp, _ip, _ip, _ip, ctypes.POINTER(ctypes.c_long)]
+        L.ou_batch_set_shape.argtypes = [_vp, ctypes.c_int, ctypes.c_int]
+        L.ou_batch_set_graph.argtypes = [_vp, ctypes.c_int]
+        L.ou_batch_apply_device.argtypes = [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _vp]
+        L.ou_batch_sweep_device.argtypes = [_vp, ctypes.POINTER(_vp), ctypes.c_int, _vp]
+        L.ou_batch_tune.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(TuneEntry), ctypes.c_int,
+                                    _ip]
+        L.ou_batch_time_apply.argtypes = [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                          ctypes.c_int, ctypes.c_int,
+                                          ctypes.POINTER(ctypes.c_double)]
         _lib = L
     return _lib
 
@@ -437,6 +457,18 @@ class RadiativeTransfer:
             _check(got)
         return r[:got], e[:got]
 
+    # --- sweep workgroup shape ---------------------------------------------------
+    @property
+    def sweep_shape(self):
+        """(waves per workgroup, direction groups per octant)."""
+        w, g = ctypes.c_int(), ctypes.c_int()
+        _check(lib().ou_sweep_shape(self._plan, ctypes.byref(w), ctypes.byref(g)))
+        return w.value, g.value
+
+    @sweep_shape.setter
+    def sweep_shape(self, shape):
+        _check(lib().ou_set_sweep_shape(self._plan, int(shape[0]), int(shape[1])))
+
     # --- device API --------------------------------------------------------------
     def apply_device(self, d_x, d_y):
         _check(lib().ou_apply_device(self._plan, d_x.ptr, d_y.ptr, None))
@@ -453,6 +485,90 @@ class RadiativeTransfer:
         if getattr(self, "_plan", None):
             lib().ou_plan_destroy(self._plan)
             self._plan = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _ptr_table(bufs):
+    arr = (_vp * len(bufs))()
+    for i, b in enumerate(bufs):
+        arr[i] = b.ptr if isinstance(b, DeviceBuffer) else b
+    return arr
+
+
+class Batch:
+    """Many RadiativeTransfer operators swept as one (ou_batch_*): merged
+    wavefront levels, one launch per element-wise stage for all members, the
+    whole sequence captured as a HIP graph and replayed.  waves/dir_groups
+    None = autotuned.  The operators must outlive the batch."""
+
+    def __init__(self, ops, waves=None, dir_groups=None):
+        self.ops = list(ops)
+        plans = (_vp * len(self.ops))(*[op._plan for op in self.ops])
+        self._b = ctypes.c_void_p()
+        _check(lib().ou_batch_create(plans, len(self.ops), int(waves or 0),
+                                     int(dir_groups or 0), ctypes.byref(self._b)))
+
+    def info(self):
+        w, g, nl, nc = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_long()
+        _check(lib().ou_batch_info(self._b, ctypes.byref(w), ctypes.byref(g), ctypes.byref(nl),
+                                   ctypes.byref(nc)))
+        return {"waves": w.value, "dir_groups": g.value, "n_levels": nl.value,
+                "n_chunks": nc.value}
+
+    def set_shape(self, waves, dir_groups):
+        _check(lib().ou_batch_set_shape(self._b, int(waves), int(dir_groups)))
+
+    def set_graph(self, on):
+        _check(lib().ou_batch_set_graph(self._b, int(bool(on))))
+
+    def apply_device(self, xs, ys, stream=None):
+        _check(lib().ou_batch_apply_device(self._b, _ptr_table(xs), _ptr_table(ys), stream))
+
+    def sweep_device(self, ys, with_source=False, stream=None):
+        _check(lib().ou_batch_sweep_device(self._b, _ptr_table(ys), int(bool(with_source)),
+                                           stream))
+
+    def apply(self, xs):
+        """Host convenience: list of numpy vectors in, list out."""
+        dx = [DeviceBuffer(x.size * 8) for x in xs]
+        dy = [DeviceBuffer(x.size * 8) for x in xs]
+        try:
+            for b, x in zip(dx, xs):
+                b.upload(np.ascontiguousarray(x, dtype=np.float64))
+            self.apply_device(dx, dy)
+            _check(lib().ou_device_synchronize())
+            out = []
+            for b, x in zip(dy, xs):
+                y = np.empty(x.size)
+                b.download(y)
+                out.append(y)
+            return out
+        finally:
+            for b in dx + dy:
+                b.free()
+
+    def tune(self, reps=3):
+        entries = (TuneEntry * 64)()
+        n = ctypes.c_int()
+        _check(lib().ou_batch_tune(self._b, int(reps), entries, 64, ctypes.byref(n)))
+        return [{"waves": e.waves, "dir_groups": e.dir_groups, "ms": e.ms}
+                for e in entries[:min(n.value, 64)]]
+
+    def time_apply(self, xs, ys, warmup, steps):
+        ms = ctypes.c_double()
+        _check(lib().ou_batch_time_apply(self._b, _ptr_table(xs), _ptr_table(ys), int(warmup),
+                                         int(steps), ctypes.byref(ms)))
+        return ms.value
+
+    def close(self):
+        if getattr(self, "_b", None):
+            lib().ou_batch_destroy(self._b)
+            self._b = None
 
     def __del__(self):
         try:

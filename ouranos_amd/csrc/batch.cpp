@@ -1,0 +1,401 @@
+// The persistent multi-instance launch (merged level schedule, captured HIP
+// graphs, device pointer table) and its autotuner.
+#include "batch.hpp"
+
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <map>
+#include <mutex>
+#include <set>
+#include <sstream>
+
+#include "vec.hpp"
+
+namespace ou {
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Tuning cache: key -> (waves, direction groups).  Kept for the process; when
+// OURANOS_TUNE_CACHE names a file, entries are read from it on first use and
+// every new result is appended ("<key> <waves> <groups>" per line).
+// ---------------------------------------------------------------------------
+std::mutex g_tune_mu;
+std::map<std::string, std::pair<int, int>> g_tune_cache;
+bool g_tune_loaded = false;
+
+void load_tune_cache_locked() {
+  if (g_tune_loaded) return;
+  g_tune_loaded = true;
+  const char *path = std::getenv("OURANOS_TUNE_CACHE");
+  if (!path) return;
+  std::ifstream in(path);
+  std::string line;
+  while (std::getline(in, line)) {
+    const size_t q = line.rfind(' '), p = q == std::string::npos ? q : line.rfind(' ', q - 1);
+    if (p == std::string::npos || p == 0) continue;
+    const int w = std::atoi(line.c_str() + p + 1), g = std::atoi(line.c_str() + q + 1);
+    if (w > 0 && g > 0) g_tune_cache[line.substr(0, p)] = {w, g};
+  }
+}
+
+bool tune_lookup(const std::string &key, int &w, int &g) {
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  load_tune_cache_locked();
+  auto it = g_tune_cache.find(key);
+  if (it == g_tune_cache.end()) return false;
+  w = it->second.first;
+  g = it->second.second;
+  return true;
+}
+
+void tune_store(const std::string &key, int w, int g) {
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  load_tune_cache_locked();
+  g_tune_cache[key] = {w, g};
+  if (const char *path = std::getenv("OURANOS_TUNE_CACHE")) {
+    std::ofstream out(path, std::ios::app);
+    out << key << ' ' << w << ' ' << g << '\n';
+  }
+}
+
+template <typename T>
+T *dev_upload(const std::vector<T> &v) {
+  T *p = nullptr;
+  OU_HIP(hipMalloc(&p, std::max<size_t>(1, v.size()) * sizeof(T)));
+  if (!v.empty()) OU_HIP(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  return p;
+}
+
+int max_dirs_per_octant(const Plan &pl) {
+  int m = 0;
+  for (int o = 0; o < pl.n_oct; ++o) m = std::max(m, pl.oct_off[o + 1] - pl.oct_off[o]);
+  return m;
+}
+
+}  // namespace
+
+Batch::Batch(std::vector<RadiativeTransfer *> members, int waves, int dir_groups)
+    : members_(std::move(members)) {
+  try {
+    if (members_.empty()) fail(OU_ERR_ARG, "empty batch");
+    for (RadiativeTransfer *m : members_)
+      if (!m) fail(OU_ERR_ARG, "null batch member");
+    {
+      std::set<const RadiativeTransfer *> uniq(members_.begin(), members_.end());
+      if (uniq.size() != members_.size())
+        fail(OU_ERR_ARG, "a plan appears twice in the batch (members share no state)");
+    }
+    OU_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    OU_HIP(hipEventCreateWithFlags(&io_done_, hipEventDisableTiming));
+    OU_HIP(hipHostMalloc(reinterpret_cast<void **>(&h_io_), 2 * members_.size() * sizeof(double *)));
+    OU_HIP(hipMalloc(reinterpret_cast<void **>(&d_io_), 2 * members_.size() * sizeof(double *)));
+    build_schedule();
+    if (waves > 0 && dir_groups > 0) {
+      set_shape(waves, dir_groups);
+    } else {
+      int w = 0, g = 0;
+      if (tune_lookup(tune_key(), w, g))
+        set_shape(w, g);
+      else
+        tune(3);
+    }
+  } catch (...) {
+    release();
+    throw;
+  }
+}
+
+Batch::~Batch() { release(); }
+
+void Batch::release() {
+  drop_graphs();
+  for (void *p : {static_cast<void *>(d_chunks_), static_cast<void *>(d_chunk_view_),
+                  static_cast<void *>(d_views_), static_cast<void *>(d_members_),
+                  static_cast<void *>(d_io_)})
+    if (p) (void)hipFree(p);
+  d_chunks_ = nullptr;
+  d_chunk_view_ = nullptr;
+  d_views_ = nullptr;
+  d_members_ = nullptr;
+  d_io_ = nullptr;
+  if (h_io_) (void)hipHostFree(h_io_);
+  h_io_ = nullptr;
+  if (io_done_) (void)hipEventDestroy(io_done_);
+  io_done_ = nullptr;
+  if (stream_) (void)hipStreamDestroy(stream_);
+  stream_ = nullptr;
+}
+
+void Batch::build_schedule() {
+  const Plan &p0 = members_[0]->plan();
+  bl_.n_members = size();
+  bl_.dim = p0.dim;
+  bl_.p = p0.p;
+  bl_.N = p0.N;
+  bl_.mm = p0.mm;
+  bl_.precision = p0.precision;
+  int n_levels = 0;
+  long max_elems = 0;
+  std::vector<std::vector<Chunk>> host(members_.size());
+  for (size_t u = 0; u < members_.size(); ++u) {
+    const Plan &pl = members_[u]->plan();
+    if (pl.dim != p0.dim || pl.p != p0.p || pl.mm != p0.mm || pl.precision != p0.precision)
+      fail(OU_ERR_ARG, "batch members must share dimension, element order, moment count "
+                       "and precision");
+    if (pl.group != 0) fail(OU_ERR_UNSUPPORTED, "batch members must use energy group 0");
+    n_levels = std::max(n_levels, pl.n_levels);
+    max_elems = std::max({max_elems, pl.vec_size, pl.n_cells * pl.N});
+    for (int a = 0; a < pl.dim; ++a)
+      max_elems = std::max(max_elems, long(pl.F) * pl.plane_size[a] * pl.n_dir);
+    host[u].resize(size_t(pl.n_chunks));
+    if (pl.n_chunks > 0)
+      OU_HIP(hipMemcpy(host[u].data(), pl.d_chunks, size_t(pl.n_chunks) * sizeof(Chunk),
+                       hipMemcpyDeviceToHost));
+  }
+  std::vector<Chunk> chunks;
+  std::vector<int> view;
+  bl_.level_off.assign(1, 0);
+  for (int l = 0; l < n_levels; ++l) {
+    for (size_t u = 0; u < members_.size(); ++u) {
+      const Plan &pl = members_[u]->plan();
+      if (l >= pl.n_levels) continue;
+      for (int c = pl.level_chunk_off[l]; c < pl.level_chunk_off[l + 1]; ++c) {
+        chunks.push_back(host[u][size_t(c)]);
+        view.push_back(int(u));
+      }
+    }
+    bl_.level_off.push_back(int(chunks.size()));
+  }
+  bl_.max_elems = max_elems;
+  d_chunks_ = dev_upload(chunks);
+  d_chunk_view_ = dev_upload(view);
+  bl_.chunks = d_chunks_;
+  bl_.chunk_view = d_chunk_view_;
+  bl_.waves = p0.waves;
+  bl_.dir_groups = p0.dir_groups;
+}
+
+void Batch::build_views() {
+  const size_t n = members_.size();
+  std::vector<SweepView> views(2 * n);
+  std::vector<MemberView> mv(n);
+  seen_partial_.assign(n, nullptr);
+  for (size_t u = 0; u < n; ++u) {
+    const Plan &pl = members_[u]->plan();
+    views[u] = pl.view();
+    views[n + u] = views[u];
+    views[n + u].S0 = pl.d_S0;
+    MemberView &m = mv[u];
+    m.n_cells = pl.n_cells;
+    m.n_dofs = pl.n_dofs;
+    m.vec_size = pl.vec_size;
+    m.n_mom = pl.n_mom;
+    m.L = pl.L;
+    m.n_dir = pl.n_dir;
+    m.n_parts = pl.n_oct * pl.dir_groups;
+    m.mat = pl.d_mat;
+    m.sigma_s = pl.d_sigma_s;
+    m.mom_l = pl.d_mom_l;
+    m.mass = pl.d_mass;
+    m.source = pl.d_source;
+    m.rowsum = pl.d_rowsum;
+    m.s0_factor = 1.0 / (pl.ws * pl.M2D[0]);
+    m.S = pl.d_S;
+    m.S0 = pl.d_S0;
+    for (int a = 0; a < 3; ++a) {
+      m.trace[a] = pl.d_trace[a];
+      m.trace_per_dir[a] = a < pl.dim ? long(pl.F) * pl.plane_size[a] : 0;
+    }
+    m.inflow = pl.d_inflow;
+    m.partial = pl.d_partial;
+    seen_partial_[u] = pl.d_partial;
+  }
+  if (!d_views_) OU_HIP(hipMalloc(reinterpret_cast<void **>(&d_views_), 2 * n * sizeof(SweepView)));
+  if (!d_members_) OU_HIP(hipMalloc(reinterpret_cast<void **>(&d_members_), n * sizeof(MemberView)));
+  // The tables may be read by work still in flight on any stream.
+  OU_HIP(hipDeviceSynchronize());
+  OU_HIP(hipMemcpy(d_views_, views.data(), 2 * n * sizeof(SweepView), hipMemcpyHostToDevice));
+  OU_HIP(hipMemcpy(d_members_, mv.data(), n * sizeof(MemberView), hipMemcpyHostToDevice));
+  bl_.views = d_views_;
+  bl_.views_src = d_views_ + n;
+  bl_.members = d_members_;
+}
+
+void Batch::drop_graphs() {
+  for (hipGraphExec_t &e : exec_)
+    if (e) {
+      (void)hipGraphExecDestroy(e);
+      e = nullptr;
+    }
+}
+
+void Batch::set_shape(int waves, int dir_groups) {
+  for (RadiativeTransfer *m : members_) set_sweep_shape(m->plan(), waves, dir_groups);
+  bl_.waves = waves;
+  bl_.dir_groups = dir_groups;
+  batch_prepare(bl_);
+  drop_graphs();
+  build_views();
+}
+
+// A member plan may have been reshaped (or its partial moments reallocated)
+// through another path since the views were written: restore the batch's
+// shape and rewrite the tables.
+void Batch::ensure_current() {
+  bool stale = d_views_ == nullptr;
+  for (size_t u = 0; u < members_.size(); ++u) {
+    const Plan &pl = members_[u]->plan();
+    if (pl.waves != bl_.waves || pl.dir_groups != bl_.dir_groups ||
+        (u < seen_partial_.size() && pl.d_partial != seen_partial_[u]))
+      stale = true;
+  }
+  if (stale) set_shape(bl_.waves, bl_.dir_groups);
+}
+
+void Batch::upload_io(const double *const *xs, double *const *ys, hipStream_t s) {
+  const size_t n = members_.size();
+  if (!ys) fail(OU_ERR_ARG, "null output pointer table");
+  std::set<const double *> outs;
+  for (size_t u = 0; u < n; ++u) {
+    if (!ys[u]) fail(OU_ERR_ARG, "null output vector in the batch");
+    if (xs && !xs[u]) fail(OU_ERR_ARG, "null input vector in the batch");
+    if (xs && xs[u] == ys[u]) fail(OU_ERR_ARG, "x and y must be distinct vectors");
+    if (!outs.insert(ys[u]).second) fail(OU_ERR_ARG, "two members share an output vector");
+  }
+  // The pinned table is rewritten only after the previous copy has run.
+  if (io_pending_) OU_HIP(hipEventSynchronize(io_done_));
+  for (size_t u = 0; u < n; ++u) {
+    h_io_[u] = xs ? xs[u] : nullptr;
+    h_io_[n + u] = ys[u];
+  }
+  OU_HIP(hipMemcpyAsync(d_io_, h_io_, 2 * n * sizeof(double *), hipMemcpyHostToDevice, s));
+  OU_HIP(hipEventRecord(io_done_, s));
+  io_pending_ = true;
+}
+
+void Batch::enqueue(int op, hipStream_t s) {
+  const double *const *xs = d_io_;
+  double *const *ys = const_cast<double *const *>(d_io_ + members_.size());
+  if (op == 0)
+    batch_apply(bl_, xs, ys, s);
+  else
+    batch_sweep(bl_, ys, op == 2, s);
+}
+
+void Batch::run(int op, hipStream_t s) {
+  if (!use_graph_) {
+    enqueue(op, s);
+    return;
+  }
+  if (!exec_[op]) {
+    hipGraph_t g = nullptr;
+    OU_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+    try {
+      enqueue(op, stream_);
+    } catch (...) {
+      (void)hipStreamEndCapture(stream_, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    OU_HIP(hipStreamEndCapture(stream_, &g));
+    const hipError_t e = hipGraphInstantiate(&exec_[op], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    OU_HIP(e);
+  }
+  OU_HIP(hipGraphLaunch(exec_[op], s));
+}
+
+void Batch::apply(const double *const *d_xs, double *const *d_ys, hipStream_t s) {
+  if (!d_xs) fail(OU_ERR_ARG, "null input pointer table");
+  s = s ? s : stream_;
+  ensure_current();
+  upload_io(d_xs, d_ys, s);
+  run(0, s);
+}
+
+void Batch::sweep(double *const *d_ys, bool with_source, hipStream_t s) {
+  s = s ? s : stream_;
+  ensure_current();
+  upload_io(nullptr, d_ys, s);
+  run(with_source ? 2 : 1, s);
+}
+
+std::string Batch::tune_key() const {
+  const Plan &p0 = members_[0]->plan();
+  long cells = 0;
+  for (const RadiativeTransfer *m : members_) cells += m->plan().n_cells;
+  std::ostringstream k;
+  k << "dim=" << p0.dim << ",p=" << p0.p << ",mm=" << p0.mm << ",prec=" << p0.precision
+    << ",dirs/oct=" << max_dirs_per_octant(p0) << ",members=" << members_.size()
+    << ",cells=" << cells << ",levels=" << n_levels();
+  return k.str();
+}
+
+std::vector<TuneEntry> Batch::tune(int reps) {
+  if (reps < 1) reps = 1;
+  const Plan &p0 = members_[0]->plan();
+  const int max_g = std::min(4, std::max(1, max_dirs_per_octant(p0)));
+  // Scratch vectors: one shared input, one output per member.
+  long max_vec = 0, sum_vec = 0;
+  for (const RadiativeTransfer *m : members_) {
+    max_vec = std::max(max_vec, m->plan().vec_size);
+    sum_vec += m->plan().vec_size;
+  }
+  double *x = nullptr, *y = nullptr;
+  OU_HIP(hipMalloc(&x, size_t(max_vec) * 8));
+  if (hipMalloc(&y, size_t(sum_vec) * 8) != hipSuccess) {
+    (void)hipGetLastError();
+    (void)hipFree(x);
+    fail(OU_ERR_NOMEM, "autotuner scratch allocation failed");
+  }
+  std::vector<const double *> xs(members_.size(), x);
+  std::vector<double *> ys(members_.size());
+  for (size_t u = 0, off = 0; u < members_.size(); off += members_[u]->plan().vec_size, ++u)
+    ys[u] = y + off;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  std::vector<TuneEntry> out;
+  try {
+    vec_fill(max_vec, 1.0, x, stream_);
+    OU_HIP(hipEventCreate(&e0));
+    OU_HIP(hipEventCreate(&e1));
+    for (int w : {2, 4, 6, 8, 12}) {
+      if (!sweep_supported(p0.dim, p0.p, p0.mm, p0.precision, w, p0.max_lds)) continue;
+      for (int g = 1; g <= max_g; ++g) {
+        set_shape(w, g);
+        apply(xs.data(), ys.data(), stream_);  // warm-up, captures the graph
+        OU_HIP(hipStreamSynchronize(stream_));
+        OU_HIP(hipEventRecord(e0, stream_));
+        for (int r = 0; r < reps; ++r) apply(xs.data(), ys.data(), stream_);
+        OU_HIP(hipEventRecord(e1, stream_));
+        OU_HIP(hipEventSynchronize(e1));
+        float ms = 0.0f;
+        OU_HIP(hipEventElapsedTime(&ms, e0, e1));
+        out.push_back({w, g, double(ms) / reps});
+      }
+    }
+  } catch (...) {
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipStreamSynchronize(stream_);
+    (void)hipFree(x);
+    (void)hipFree(y);
+    throw;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  OU_HIP(hipStreamSynchronize(stream_));
+  (void)hipFree(x);
+  (void)hipFree(y);
+  if (out.empty()) fail(OU_ERR_UNSUPPORTED, "no sweep shape to tune for this batch");
+  const TuneEntry best =
+      *std::min_element(out.begin(), out.end(),
+                        [](const TuneEntry &a, const TuneEntry &b) { return a.ms < b.ms; });
+  set_shape(best.waves, best.dir_groups);
+  tune_store(tune_key(), best.waves, best.dir_groups);
+  return out;
+}
+
+}  // namespace ou

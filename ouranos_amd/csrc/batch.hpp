@@ -1,0 +1,85 @@
+// Many independent transport operators swept as one: the persistent
+// multi-instance launch and its autotuner.
+//
+// The members' wavefront levels are merged (level l of the batch holds
+// level l of every member), so a batch of any number of uneven meshes costs
+// max(levels) sweep launches instead of sum(levels), and every launch has the
+// chunks of all members to spread over the chip.  The element-wise stages
+// (scattering source, trace set-up, epilogue) are one launch each for the
+// whole batch.  The whole sequence is captured once per operation as a HIP
+// graph and replayed; the per-member vector pointers live in a device table
+// written before each replay, so one graph serves any vectors.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "radiative_transfer.hpp"
+
+namespace ou {
+
+struct TuneEntry {
+  int waves = 0, dir_groups = 0;
+  double ms = 0.0;  // mean time of one batched Apply
+};
+
+class Batch {
+ public:
+  // All members must share dimension, element order, moment count and
+  // precision.  waves / dir_groups <= 0 ask the autotuner (or its cache).
+  Batch(std::vector<RadiativeTransfer *> members, int waves, int dir_groups);
+  ~Batch();
+  Batch(const Batch &) = delete;
+  Batch &operator=(const Batch &) = delete;
+
+  int size() const { return int(members_.size()); }
+  int waves() const { return bl_.waves; }
+  int dir_groups() const { return bl_.dir_groups; }
+  int n_levels() const { return int(bl_.level_off.size()) - 1; }
+  long n_chunks() const { return bl_.level_off.back(); }
+  hipStream_t stream() const { return stream_; }
+
+  // y_i = x_i - D L^-1 M S x_i for every member i (d_xs, d_ys: host arrays of
+  // device pointers, one per member).
+  void apply(const double *const *d_xs, double *const *d_ys, hipStream_t s);
+  // y_i += D L^-1 (M2D S_i [+ M q_i/ws + inflow]) for every member.
+  void sweep(double *const *d_ys, bool with_source, hipStream_t s);
+
+  void set_shape(int waves, int dir_groups);
+  void set_use_graph(bool on) { use_graph_ = on; }
+  bool use_graph() const { return use_graph_; }
+  // Times every supported (waves, direction groups) shape on this batch,
+  // keeps the fastest and records it in the tuning cache.
+  std::vector<TuneEntry> tune(int reps);
+  // The cache key of this batch's shape class.
+  std::string tune_key() const;
+
+ private:
+  void build_schedule();
+  void build_views();
+  void ensure_current();
+  void upload_io(const double *const *xs, double *const *ys, hipStream_t s);
+  void run(int op, hipStream_t s);
+  void enqueue(int op, hipStream_t s);
+  void drop_graphs();
+  void release();
+
+  std::vector<RadiativeTransfer *> members_;
+  BatchLaunch bl_;
+  hipStream_t stream_ = nullptr;
+  Chunk *d_chunks_ = nullptr;
+  int *d_chunk_view_ = nullptr;
+  SweepView *d_views_ = nullptr;     // [2][members]: plain, with source
+  MemberView *d_members_ = nullptr;  // [members]
+  const double **d_io_ = nullptr;    // [2][members]: x pointers, then y pointers
+  const double **h_io_ = nullptr;    // pinned staging of the table
+  hipEvent_t io_done_ = nullptr;
+  bool io_pending_ = false;
+  std::vector<double *> seen_partial_;
+  bool use_graph_ = true;
+  hipGraphExec_t exec_[3] = {nullptr, nullptr, nullptr};  // apply, sweep, sweep + source
+};
+
+}  // namespace ou

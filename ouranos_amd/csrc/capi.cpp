@@ -5,6 +5,7 @@
 #include <cstring>
 #include <string>
 
+#include "batch.hpp"
 #include "fe.hpp"
 #include "learned.hpp"
 #include "quadrature.hpp"
@@ -28,6 +29,10 @@ struct ou_unet {
 
 struct ou_quadrature {
   ou::Quadrature q;
+};
+
+struct ou_batch {
+  std::unique_ptr<ou::Batch> b;
 };
 
 namespace ou {
@@ -59,6 +64,11 @@ int guarded(F &&f) {
 ou::Plan &plan_of(ou_plan *p) {
   if (!p || !p->rt) ou::fail(OU_ERR_ARG, "null plan");
   return p->rt->plan();
+}
+
+ou::Batch &batch_of(const ou_batch *b) {
+  if (!b || !b->b) ou::fail(OU_ERR_ARG, "null batch");
+  return *b->b;
 }
 
 hipStream_t stream_of(ou_plan *p, void *s) {
@@ -432,6 +442,133 @@ int ou_time_apply(ou_plan *plan, const double *d_x, double *d_y, int warmup, int
     timing->ms_other = ms_total - ms_sweep;
     timing->n_sweep_launches = launches;
     timing->applications = steps;
+    return int(OU_OK);
+  });
+}
+
+// ---------------------------------------------------------------------------
+// Sweep shape and the batched launch
+// ---------------------------------------------------------------------------
+
+int ou_set_sweep_shape(ou_plan *plan, int waves, int dir_groups) {
+  return guarded([&] {
+    ou::set_sweep_shape(plan_of(plan), waves, dir_groups);
+    return int(OU_OK);
+  });
+}
+
+int ou_sweep_shape(const ou_plan *plan, int *waves, int *dir_groups) {
+  return guarded([&] {
+    const ou::Plan &pl = plan_of(const_cast<ou_plan *>(plan));
+    if (waves) *waves = pl.waves;
+    if (dir_groups) *dir_groups = pl.dir_groups;
+    return int(OU_OK);
+  });
+}
+
+int ou_batch_create(ou_plan *const *plans, int n_plans, int waves, int dir_groups,
+                    ou_batch **batch) {
+  return guarded([&] {
+    if (!batch) ou::fail(OU_ERR_ARG, "null output pointer");
+    if (!plans || n_plans < 1) ou::fail(OU_ERR_ARG, "a batch needs at least one plan");
+    std::vector<ou::RadiativeTransfer *> members(static_cast<size_t>(n_plans));
+    for (int i = 0; i < n_plans; ++i) {
+      plan_of(plans[i]);
+      members[size_t(i)] = plans[i]->rt.get();
+    }
+    auto *b = new ou_batch();
+    try {
+      b->b.reset(new ou::Batch(std::move(members), waves, dir_groups));
+    } catch (...) {
+      delete b;
+      throw;
+    }
+    *batch = b;
+    return int(OU_OK);
+  });
+}
+
+void ou_batch_destroy(ou_batch *batch) { delete batch; }
+
+int ou_batch_info(const ou_batch *batch, int *waves, int *dir_groups, int *n_levels,
+                  long *n_chunks) {
+  return guarded([&] {
+    const ou::Batch &b = batch_of(batch);
+    if (waves) *waves = b.waves();
+    if (dir_groups) *dir_groups = b.dir_groups();
+    if (n_levels) *n_levels = b.n_levels();
+    if (n_chunks) *n_chunks = b.n_chunks();
+    return int(OU_OK);
+  });
+}
+
+int ou_batch_set_shape(ou_batch *batch, int waves, int dir_groups) {
+  return guarded([&] {
+    batch_of(batch).set_shape(waves, dir_groups);
+    return int(OU_OK);
+  });
+}
+
+int ou_batch_set_graph(ou_batch *batch, int use_graph) {
+  return guarded([&] {
+    batch_of(batch).set_use_graph(use_graph != 0);
+    return int(OU_OK);
+  });
+}
+
+int ou_batch_apply_device(ou_batch *batch, const double *const *d_x, double *const *d_y,
+                          void *stream) {
+  return guarded([&] {
+    batch_of(batch).apply(d_x, d_y, static_cast<hipStream_t>(stream));
+    return int(OU_OK);
+  });
+}
+
+int ou_batch_sweep_device(ou_batch *batch, double *const *d_y, int with_source,
+                          void *stream) {
+  return guarded([&] {
+    batch_of(batch).sweep(d_y, with_source != 0, static_cast<hipStream_t>(stream));
+    return int(OU_OK);
+  });
+}
+
+int ou_batch_tune(ou_batch *batch, int reps, ou_tune_entry *entries, int max_entries,
+                  int *n_entries) {
+  return guarded([&] {
+    const std::vector<ou::TuneEntry> t = batch_of(batch).tune(reps);
+    const int n = std::min<int>(int(t.size()), std::max(0, max_entries));
+    for (int i = 0; i < n && entries; ++i)
+      entries[i] = ou_tune_entry{t[size_t(i)].waves, t[size_t(i)].dir_groups, t[size_t(i)].ms};
+    if (n_entries) *n_entries = int(t.size());
+    return int(OU_OK);
+  });
+}
+
+int ou_batch_time_apply(ou_batch *batch, const double *const *d_x, double *const *d_y,
+                        int warmup, int steps, double *ms) {
+  return guarded([&] {
+    ou::Batch &b = batch_of(batch);
+    if (!ms || steps < 1 || warmup < 0) ou::fail(OU_ERR_ARG, "bad timing arguments");
+    hipStream_t s = b.stream();
+    for (int i = 0; i < warmup; ++i) b.apply(d_x, d_y, s);
+    OU_HIP(hipStreamSynchronize(s));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    OU_HIP(hipEventCreate(&e0));
+    struct Ev {
+      hipEvent_t *a, *b;
+      ~Ev() {
+        if (*a) (void)hipEventDestroy(*a);
+        if (*b) (void)hipEventDestroy(*b);
+      }
+    } guard{&e0, &e1};
+    OU_HIP(hipEventCreate(&e1));
+    OU_HIP(hipEventRecord(e0, s));
+    for (int i = 0; i < steps; ++i) b.apply(d_x, d_y, s);
+    OU_HIP(hipEventRecord(e1, s));
+    OU_HIP(hipEventSynchronize(e1));
+    float t = 0.0f;
+    OU_HIP(hipEventElapsedTime(&t, e0, e1));
+    *ms = double(t) / steps;
     return int(OU_OK);
   });
 }

@@ -385,4 +385,21 @@ Plan *create_plan(const ou_problem_desc *desc) {
   }
 }
 
+void set_sweep_shape(Plan &pl, int waves, int dir_groups) {
+  if (dir_groups < 1 || dir_groups > kMaxDirGroups)
+    fail(OU_ERR_ARG, "direction groups per octant must be in [1, 8]");
+  if (!sweep_supported(pl.dim, pl.p, pl.mm, pl.precision, waves, pl.max_lds))
+    fail(OU_ERR_UNSUPPORTED, "no sweep kernel with this many waves for this element");
+  if (dir_groups > pl.partial_groups) {
+    const size_t bytes = size_t(pl.n_oct) * dir_groups * pl.vec_size * 8;
+    OU_HIP(hipDeviceSynchronize());
+    void *fresh = dev_alloc(bytes);
+    (void)hipFree(pl.d_partial);
+    pl.d_partial = static_cast<double *>(fresh);
+    pl.partial_groups = dir_groups;
+  }
+  pl.waves = waves;
+  pl.dir_groups = dir_groups;
+}
+
 }  // namespace ou

@@ -10,6 +10,8 @@
 
 namespace ou {
 
+constexpr int kMaxDirGroups = 8;
+
 // One chunk: up to 64 cells of one sweep level of one octant, all with the
 // same material (so every lane of a wave uses the same inverse matrix).
 struct Chunk {
@@ -47,9 +49,10 @@ struct SweepView {
   // use consecutive slots on every axis).
   void *trace[3];
   long plane_size[3];
-  double *partial;         // [oct][m][dof]
+  double *partial;         // [oct][group][m][dof]
   const Chunk *chunks;
   const int *cells;
+  int dir_groups;          // workgroups per (chunk, octant), each a range of directions
 };
 
 struct Plan {
@@ -69,6 +72,8 @@ struct Plan {
 
   // schedule
   int n_oct = 4, n_levels = 0, waves = 4, mm = 1;
+  int dir_groups = 1;        // direction groups per octant (one workgroup each)
+  int partial_groups = 1;    // groups d_partial is allocated for
   std::vector<int> oct_dirs, oct_off, dir_oct;
   int n_cls = 0;
   std::vector<int> dir_cls;          // [dir] inverse class
@@ -103,8 +108,45 @@ struct Plan {
   SweepView view() const;
 };
 
+// One member of a batch, as the batched element-wise kernels see it.
+struct MemberView {
+  long n_cells, n_dofs, vec_size;
+  int n_mom, L, n_dir, n_parts;  // n_parts = octants x direction groups
+  const int *mat;
+  const double *sigma_s;
+  const int *mom_l;
+  const double *mass;
+  const double *source;
+  const double *rowsum;
+  double s0_factor;
+  void *S, *S0;
+  void *trace[3];
+  long trace_per_dir[3];
+  const double *inflow;
+  const double *partial;
+};
+
+// A batch's merged level schedule on the device: level l of the batch is
+// chunks [level_off[l], level_off[l+1]) of `chunks`, chunk i belonging to the
+// member whose view is views[chunk_view[i]].
+struct BatchLaunch {
+  int n_members = 0, dim = 3, p = 1, N = 8, mm = 1, precision = OU_FP32;
+  int waves = 4, dir_groups = 1;
+  std::vector<int> level_off;  // host
+  const Chunk *chunks = nullptr;
+  const int *chunk_view = nullptr;
+  const SweepView *views = nullptr;      // S0 = null (Apply, plain sweep)
+  const SweepView *views_src = nullptr;  // S0 = member source (sweep with source)
+  const MemberView *members = nullptr;
+  long max_elems = 0;                    // largest element count of any stage / member
+};
+
 Plan *create_plan(const ou_problem_desc *desc);
 bool sweep_supported(int dim, int p, int mm, int precision, int waves, int max_lds);
+// Sets the sweep workgroup shape (waves per workgroup, direction groups per
+// octant).  Growing the group count reallocates the partial moments after a
+// device synchronisation.
+void set_sweep_shape(Plan &pl, int waves, int dir_groups);
 
 // Launch sequences (stream-ordered, no host synchronisation).
 void launch_build_inverses(Plan &pl);
@@ -119,5 +161,14 @@ void launch_epilogue(Plan &pl, const double *d_x, double *d_y, bool apply,
 // Whole operations.
 void apply_device(Plan &pl, const double *d_x, double *d_y, hipStream_t s);
 void sweep_device(Plan &pl, double *d_y, bool with_source, hipStream_t s);
+
+// Batched operations over every member (stream-ordered; capturable: kernel
+// launches only).  d_xs / d_ys are device arrays of per-member vector
+// pointers.  batch_prepare sets kernel attributes and must run outside
+// stream capture before the first batched launch of a shape.
+void batch_prepare(const BatchLaunch &b);
+void batch_apply(const BatchLaunch &b, const double *const *d_xs, double *const *d_ys,
+                 hipStream_t s);
+void batch_sweep(const BatchLaunch &b, double *const *d_ys, bool with_source, hipStream_t s);
 
 }  // namespace ou

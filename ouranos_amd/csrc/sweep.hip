@@ -139,12 +139,12 @@ __global__ __launch_bounds__(64) void k_build_inverses(
 // Scattering source S_m|K = sigma_{s,l(m)} (M x_m|K).
 // ------------------------------------------------------------------------
 template <typename T, int N>
-__global__ __launch_bounds__(256) void k_scatter_source(
-    long n_cells, int n_mom, long n_dofs, const double *__restrict__ x,
-    const int *__restrict__ mat, const double *__restrict__ sigma_s, int L,
-    const int *__restrict__ mom_l, const double *Mm, T *__restrict__ S) {
-  const long gid = long(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (gid >= n_cells * n_mom) return;
+__device__ __forceinline__ void scatter_one(long gid, long n_cells, long n_dofs,
+                                            const double *__restrict__ x,
+                                            const int *__restrict__ mat,
+                                            const double *__restrict__ sigma_s, int L,
+                                            const int *__restrict__ mom_l, const double *Mm,
+                                            T *__restrict__ S) {
   const int m = int(gid / n_cells);
   const long c = gid - long(m) * n_cells;
   const int l = mom_l[m];
@@ -164,17 +164,35 @@ __global__ __launch_bounds__(256) void k_scatter_source(
   }
 }
 
+template <typename T, int N>
+__global__ __launch_bounds__(256) void k_scatter_source(
+    long n_cells, int n_mom, long n_dofs, const double *__restrict__ x,
+    const int *__restrict__ mat, const double *__restrict__ sigma_s, int L,
+    const int *__restrict__ mom_l, const double *Mm, T *__restrict__ S) {
+  const long gid = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (gid >= n_cells * n_mom) return;
+  scatter_one<T, N>(gid, n_cells, n_dofs, x, mat, sigma_s, L, mom_l, Mm, S);
+}
+
 // Moment 0 of the source sweep: S_0 + (M 1) q / (ws M2D[.,0]), so that
 // M2D[d,0] * S0 adds the volumetric source M q/ws of RadiativeTransfer.cc:914-925.
+template <typename T, int N>
+__device__ __forceinline__ void source0_one(long gid, const T *__restrict__ S,
+                                            const double *__restrict__ q,
+                                            const double *__restrict__ rowsum, double factor,
+                                            T *__restrict__ S0) {
+  const long c = gid / N;
+  const int i = int(gid - c * N);
+  S0[gid] = T(double(S[gid]) + rowsum[i] * (q[c] * factor));
+}
+
 template <typename T, int N>
 __global__ __launch_bounds__(256) void k_source_moment0(
     long n_cells, const T *__restrict__ S, const double *__restrict__ q,
     const double *__restrict__ rowsum, double factor, T *__restrict__ S0) {
   const long gid = long(blockIdx.x) * blockDim.x + threadIdx.x;
   if (gid >= n_cells * N) return;
-  const long c = gid / N;
-  const int i = int(gid - c * N);
-  S0[gid] = T(double(S[gid]) + rowsum[i] * (q[c] * factor));
+  source0_one<T, N>(gid, S, q, rowsum, factor, S0);
 }
 
 // ------------------------------------------------------------------------
@@ -240,16 +258,87 @@ __global__ __launch_bounds__(64) void k_ainv_to_mfma(const float *__restrict__ a
   }
 }
 
-// y = x - sum_o partial_o (Apply) or y += sum_o partial_o (sweep).
-__global__ __launch_bounds__(256) void k_epilogue(long n, int n_oct, long stride,
+// y = x - sum_o partial_o (Apply) or y += sum_o partial_o (sweep), summed
+// over the n_parts = octants x direction groups partial moment vectors.
+__device__ __forceinline__ void epilogue_one(long gid, int n_parts, long stride,
+                                             const double *__restrict__ partial,
+                                             const double *__restrict__ x,
+                                             double *__restrict__ y, int apply) {
+  double s = 0.0;
+  for (int o = 0; o < n_parts; ++o) s += partial[o * stride + gid];
+  y[gid] = apply ? x[gid] - s : y[gid] + s;
+}
+
+__global__ __launch_bounds__(256) void k_epilogue(long n, int n_parts, long stride,
                                                   const double *__restrict__ partial,
                                                   const double *__restrict__ x,
                                                   double *__restrict__ y, int apply) {
   const long gid = long(blockIdx.x) * blockDim.x + threadIdx.x;
   if (gid >= n) return;
-  double s = 0.0;
-  for (int o = 0; o < n_oct; ++o) s += partial[o * stride + gid];
-  y[gid] = apply ? x[gid] - s : y[gid] + s;
+  epilogue_one(gid, n_parts, stride, partial, x, y, apply);
+}
+
+// ------------------------------------------------------------------------
+// Batched element-wise stages: one launch covers every member of a batch;
+// each block walks the members and strides over each member's elements.
+// The per-element arithmetic is the single-plan kernels' own.
+// ------------------------------------------------------------------------
+template <typename T, int N>
+__global__ __launch_bounds__(256) void k_batch_scatter(const MemberView *__restrict__ mv,
+                                                       int n_mem,
+                                                       const double *const *__restrict__ xs) {
+  const long stride = long(gridDim.x) * blockDim.x;
+  for (int u = 0; u < n_mem; ++u) {
+    const MemberView &m = mv[u];
+    const long n = m.n_cells * m.n_mom;
+    for (long gid = long(blockIdx.x) * blockDim.x + threadIdx.x; gid < n; gid += stride)
+      scatter_one<T, N>(gid, m.n_cells, m.n_dofs, xs[u], m.mat, m.sigma_s, m.L, m.mom_l,
+                        m.mass, static_cast<T *>(m.S));
+  }
+}
+
+template <typename T, int N>
+__global__ __launch_bounds__(256) void k_batch_source0(const MemberView *__restrict__ mv,
+                                                       int n_mem) {
+  const long stride = long(gridDim.x) * blockDim.x;
+  for (int u = 0; u < n_mem; ++u) {
+    const MemberView &m = mv[u];
+    const long n = m.n_cells * N;
+    for (long gid = long(blockIdx.x) * blockDim.x + threadIdx.x; gid < n; gid += stride)
+      source0_one<T, N>(gid, static_cast<const T *>(m.S), m.source, m.rowsum, m.s0_factor,
+                        static_cast<T *>(m.S0));
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_batch_trace_init(const MemberView *__restrict__ mv,
+                                                          int n_mem, int dim, int with_source) {
+  const long stride = long(gridDim.x) * blockDim.x;
+  for (int u = 0; u < n_mem; ++u) {
+    const MemberView &m = mv[u];
+    for (int a = 0; a < dim; ++a) {
+      T *tr = static_cast<T *>(m.trace[a]);
+      const long per_dir = m.trace_per_dir[a], total = per_dir * m.n_dir;
+      for (long gid = long(blockIdx.x) * blockDim.x + threadIdx.x; gid < total; gid += stride)
+        tr[gid] = with_source ? T(m.inflow[(gid / per_dir) * 3 + a]) : T(0);
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_batch_epilogue(const MemberView *__restrict__ mv,
+                                                        int n_mem,
+                                                        const double *const *__restrict__ xs,
+                                                        double *const *__restrict__ ys,
+                                                        int apply) {
+  const long stride = long(gridDim.x) * blockDim.x;
+  for (int u = 0; u < n_mem; ++u) {
+    const MemberView &m = mv[u];
+    const double *x = apply ? xs[u] : nullptr;
+    double *y = ys[u];
+    for (long gid = long(blockIdx.x) * blockDim.x + threadIdx.x; gid < m.vec_size;
+         gid += stride)
+      epilogue_one(gid, m.n_parts, m.vec_size, m.partial, x, y, apply);
+  }
 }
 
 inline unsigned grid_for(long n, int block) { return unsigned((n + block - 1) / block); }
@@ -257,8 +346,19 @@ inline unsigned grid_for(long n, int block) { return unsigned((n + block - 1) / 
 // ------------------------------------------------------------------------
 // Dispatch tables.
 // ------------------------------------------------------------------------
-using LevelLauncher = void (*)(const Plan &, const SweepView &, hipStream_t, hipEvent_t,
-                               hipEvent_t);
+// The level kernels of one instantiation: the single-plan launch sequence
+// and the batched one (batched: `launch` false only sets the attributes).
+struct LevelKernels {
+  void (*single)(const Plan &, const SweepView &, hipStream_t, hipEvent_t, hipEvent_t);
+  void (*batch)(const BatchLaunch &, const SweepView *, hipStream_t, bool launch);
+};
+
+template <typename T, int DIM, int P, int MM, int W>
+void set_lds_attr(const void *kern) {
+  using C = SweepCfg<T, DIM, P, MM, W>;
+  OU_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             int(C::lds_bytes)));
+}
 
 template <typename T, int DIM, int P, int MM, int W>
 void launch_levels_impl(const Plan &pl, const SweepView &v, hipStream_t s,
@@ -267,45 +367,77 @@ void launch_levels_impl(const Plan &pl, const SweepView &v, hipStream_t s,
   auto kern = k_sweep_level<T, DIM, P, MM, W>;
   static bool attr_set = false;
   if (!attr_set) {
-    OU_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
-                               hipFuncAttributeMaxDynamicSharedMemorySize,
-                               int(C::lds_bytes)));
+    set_lds_attr<T, DIM, P, MM, W>(reinterpret_cast<const void *>(kern));
     attr_set = true;
   }
   if (before) OU_HIP(hipEventRecord(before, s));
+  const unsigned groups = unsigned(pl.dir_groups);
   for (int l = 0; l < pl.n_levels; ++l) {
     const int b = pl.level_chunk_off[l], e = pl.level_chunk_off[l + 1];
-    if (e > b) hipLaunchKernelGGL(kern, dim3(e - b), dim3(64 * W), C::lds_bytes, s, v, b);
+    if (e > b)
+      hipLaunchKernelGGL(kern, dim3(e - b, groups), dim3(64 * W), C::lds_bytes, s, v, b);
   }
   if (after) OU_HIP(hipEventRecord(after, s));
   OU_HIP(hipGetLastError());
 }
 
+template <typename T, int DIM, int P, int MM, int W>
+void launch_batch_levels_impl(const BatchLaunch &bl, const SweepView *views, hipStream_t s,
+                              bool launch) {
+  using C = SweepCfg<T, DIM, P, MM, W>;
+  auto kern = k_sweep_level_batch<T, DIM, P, MM, W>;
+  if (!launch) {
+    set_lds_attr<T, DIM, P, MM, W>(reinterpret_cast<const void *>(kern));
+    return;
+  }
+  const unsigned groups = unsigned(bl.dir_groups);
+  const int n_levels = int(bl.level_off.size()) - 1;
+  for (int l = 0; l < n_levels; ++l) {
+    const int b = bl.level_off[l], e = bl.level_off[l + 1];
+    if (e > b)
+      hipLaunchKernelGGL(kern, dim3(e - b, groups), dim3(64 * W), C::lds_bytes, s, views,
+                         bl.chunk_view, bl.chunks, b);
+  }
+  OU_HIP(hipGetLastError());
+}
+
+template <typename T, int DIM, int P, int MM, int W>
+const LevelKernels *kernels_of() {
+  static const LevelKernels k{launch_levels_impl<T, DIM, P, MM, W>,
+                              launch_batch_levels_impl<T, DIM, P, MM, W>};
+  return &k;
+}
+
 template <typename T, int DIM, int P, int W>
-LevelLauncher pick_mm(int mm) {
+const LevelKernels *pick_mm(int mm) {
   switch (mm) {
-    case 1: return launch_levels_impl<T, DIM, P, 1, W>;
-    case 4: return launch_levels_impl<T, DIM, P, 4, W>;
-    case 10: if constexpr (W == 4) return launch_levels_impl<T, DIM, P, 10, W>; break;
-    case 16: if constexpr (W == 4) return launch_levels_impl<T, DIM, P, 16, W>; break;
+    case 1: return kernels_of<T, DIM, P, 1, W>();
+    case 4: return kernels_of<T, DIM, P, 4, W>();
+    case 10: if constexpr (W == 4) return kernels_of<T, DIM, P, 10, W>(); break;
+    case 16: if constexpr (W == 4) return kernels_of<T, DIM, P, 16, W>(); break;
   }
   return nullptr;
 }
 
+// Instantiated shapes: 4 waves everywhere; the 3D shapes with at most 4
+// moments also at the other wave counts the autotuner tries.
 template <typename T>
-LevelLauncher pick(int dim, int p, int mm, int waves) {
+const LevelKernels *pick(int dim, int p, int mm, int waves) {
   if (waves == 4) {
     if (dim == 2 && p == 1) return pick_mm<T, 2, 1, 4>(mm);
     if (dim == 2 && p == 2) return pick_mm<T, 2, 2, 4>(mm);
     if (dim == 2 && p == 3) return pick_mm<T, 2, 3, 4>(mm);
     if (dim == 3 && p == 1) return pick_mm<T, 3, 1, 4>(mm);
     if (dim == 3 && p == 2) return pick_mm<T, 3, 2, 4>(mm);
-  } else if (waves == 8) {
-    if (dim == 3 && p == 2 && mm <= 4) return pick_mm<T, 3, 2, 8>(mm);
-  } else if (waves == 12) {
-    if (dim == 3 && p == 2 && mm <= 4) return pick_mm<T, 3, 2, 12>(mm);
-  } else if (waves == 6) {
-    if (dim == 3 && p == 2 && mm <= 4) return pick_mm<T, 3, 2, 6>(mm);
+  } else if (dim == 3 && mm <= 4) {
+    if (p == 1) {
+      if (waves == 2) return pick_mm<T, 3, 1, 2>(mm);
+      if (waves == 8) return pick_mm<T, 3, 1, 8>(mm);
+    } else if (p == 2) {
+      if (waves == 6) return pick_mm<T, 3, 2, 6>(mm);
+      if (waves == 8) return pick_mm<T, 3, 2, 8>(mm);
+      if (waves == 12) return pick_mm<T, 3, 2, 12>(mm);
+    }
   }
   return nullptr;
 }
@@ -399,6 +531,7 @@ SweepView Plan::view() const {
   v.partial = d_partial;
   v.chunks = d_chunks;
   v.cells = d_cells;
+  v.dir_groups = dir_groups;
   return v;
 }
 
@@ -485,16 +618,17 @@ void launch_sweep_levels(Plan &pl, const void *S0, hipStream_t s, hipEvent_t bef
                          hipEvent_t after) {
   SweepView v = pl.view();
   v.S0 = S0;
-  LevelLauncher f = pl.precision == OU_FP32 ? pick<float>(pl.dim, pl.p, pl.mm, pl.waves)
-                                            : pick<double>(pl.dim, pl.p, pl.mm, pl.waves);
-  if (!f) fail(OU_ERR_UNSUPPORTED, "no sweep kernel for this element / moment count");
-  f(pl, v, s, before, after);
+  const LevelKernels *k = pl.precision == OU_FP32 ? pick<float>(pl.dim, pl.p, pl.mm, pl.waves)
+                                                  : pick<double>(pl.dim, pl.p, pl.mm, pl.waves);
+  if (!k) fail(OU_ERR_UNSUPPORTED, "no sweep kernel for this element / moment count");
+  k->single(pl, v, s, before, after);
 }
 
 void launch_epilogue(Plan &pl, const double *d_x, double *d_y, bool apply, hipStream_t s) {
   const long n = pl.vec_size;
-  hipLaunchKernelGGL(k_epilogue, dim3(grid_for(n, 256)), dim3(256), 0, s, n, pl.n_oct,
-                     pl.vec_size, pl.d_partial, d_x, d_y, int(apply));
+  hipLaunchKernelGGL(k_epilogue, dim3(grid_for(n, 256)), dim3(256), 0, s, n,
+                     pl.n_oct * pl.dir_groups, pl.vec_size, pl.d_partial, d_x, d_y,
+                     int(apply));
   OU_HIP(hipGetLastError());
 }
 
@@ -503,6 +637,96 @@ void apply_device(Plan &pl, const double *d_x, double *d_y, hipStream_t s) {
   launch_trace_init(pl, false, s);
   launch_sweep_levels(pl, nullptr, s);
   launch_epilogue(pl, d_x, d_y, true, s);
+}
+
+namespace {
+
+const LevelKernels *batch_kernels(const BatchLaunch &b) {
+  const LevelKernels *k = b.precision == OU_FP32 ? pick<float>(b.dim, b.p, b.mm, b.waves)
+                                                 : pick<double>(b.dim, b.p, b.mm, b.waves);
+  if (!k) fail(OU_ERR_UNSUPPORTED, "no batched sweep kernel for this shape");
+  return k;
+}
+
+unsigned batch_grid(const BatchLaunch &b) {
+  return std::max(1u, std::min(4096u, grid_for(b.max_elems, 256)));
+}
+
+template <typename T>
+void batch_scatter_impl(const BatchLaunch &b, const double *const *d_xs, hipStream_t s) {
+  const unsigned g = batch_grid(b);
+#define OU_BSCAT(NN)                                                                   \
+  case NN:                                                                             \
+    hipLaunchKernelGGL((k_batch_scatter<T, NN>), dim3(g), dim3(256), 0, s, b.members, \
+                       b.n_members, d_xs);                                             \
+    break;
+  switch (b.N) {
+    OU_BSCAT(4) OU_BSCAT(9) OU_BSCAT(16) OU_BSCAT(8) OU_BSCAT(27)
+    default: fail(OU_ERR_UNSUPPORTED, "batched scatter: unsupported element size");
+  }
+#undef OU_BSCAT
+}
+
+template <typename T>
+void batch_source0_impl(const BatchLaunch &b, hipStream_t s) {
+  const unsigned g = batch_grid(b);
+#define OU_BSRC(NN)                                                                    \
+  case NN:                                                                             \
+    hipLaunchKernelGGL((k_batch_source0<T, NN>), dim3(g), dim3(256), 0, s, b.members, \
+                       b.n_members);                                                   \
+    break;
+  switch (b.N) {
+    OU_BSRC(4) OU_BSRC(9) OU_BSRC(16) OU_BSRC(8) OU_BSRC(27)
+    default: fail(OU_ERR_UNSUPPORTED, "batched source: unsupported element size");
+  }
+#undef OU_BSRC
+}
+
+void batch_trace_init(const BatchLaunch &b, bool with_source, hipStream_t s) {
+  const unsigned g = batch_grid(b);
+  if (b.precision == OU_FP32)
+    hipLaunchKernelGGL(k_batch_trace_init<float>, dim3(g), dim3(256), 0, s, b.members,
+                       b.n_members, b.dim, int(with_source));
+  else
+    hipLaunchKernelGGL(k_batch_trace_init<double>, dim3(g), dim3(256), 0, s, b.members,
+                       b.n_members, b.dim, int(with_source));
+}
+
+void batch_epilogue(const BatchLaunch &b, const double *const *d_xs, double *const *d_ys,
+                    bool apply, hipStream_t s) {
+  hipLaunchKernelGGL(k_batch_epilogue, dim3(batch_grid(b)), dim3(256), 0, s, b.members,
+                     b.n_members, d_xs, d_ys, int(apply));
+}
+
+}  // namespace
+
+void batch_prepare(const BatchLaunch &b) { batch_kernels(b)->batch(b, nullptr, nullptr, false); }
+
+void batch_apply(const BatchLaunch &b, const double *const *d_xs, double *const *d_ys,
+                 hipStream_t s) {
+  const LevelKernels *k = batch_kernels(b);
+  if (b.precision == OU_FP32)
+    batch_scatter_impl<float>(b, d_xs, s);
+  else
+    batch_scatter_impl<double>(b, d_xs, s);
+  batch_trace_init(b, false, s);
+  k->batch(b, b.views, s, true);
+  batch_epilogue(b, d_xs, d_ys, true, s);
+  OU_HIP(hipGetLastError());
+}
+
+void batch_sweep(const BatchLaunch &b, double *const *d_ys, bool with_source, hipStream_t s) {
+  const LevelKernels *k = batch_kernels(b);
+  if (with_source) {
+    if (b.precision == OU_FP32)
+      batch_source0_impl<float>(b, s);
+    else
+      batch_source0_impl<double>(b, s);
+  }
+  batch_trace_init(b, with_source, s);
+  k->batch(b, with_source ? b.views_src : b.views, s, true);
+  batch_epilogue(b, nullptr, d_ys, false, s);
+  OU_HIP(hipGetLastError());
 }
 
 void sweep_device(Plan &pl, double *d_y, bool with_source, hipStream_t s) {

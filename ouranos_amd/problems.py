@@ -164,6 +164,33 @@ def batch_3d(n=32, block=8, seed=3, sn=12, ws=4.0 * np.pi, quad=None):
                        qb[mat], ws, meta={"seed": seed, "block": block})
 
 
+def batch_instances(n_instances=32, n_rhs=4, sizes=(16, 24, 32, 48, 64), block=8, seed=3,
+                    sn=12, ws=4.0 * np.pi, quad=None):
+    """The batched workload: n_instances independent unit cubes of n^3 Q1
+    cells, n drawn uniformly from `sizes`, each with its own block materials
+    (sigma_t log-U[0.1,50], c ~ U[0,0.95], isotropic scattering) and n_rhs
+    block sources q ~ U[0,1].  Returns n_instances * n_rhs specs, instance
+    major; the specs of one instance share everything but q.  Draw order per
+    instance: n, then (sigma_t, c) per block, then q per block per RHS."""
+    quad = quad or Quadrature(LS, 3, sn=sn, L_max=0, weight_sum=ws)
+    rng = SplitMix64(seed)
+    specs = []
+    for i in range(n_instances):
+        n = int(sizes[min(int(rng.uniform() * len(sizes)), len(sizes) - 1)])
+        mat, nb = _block_map((n, n, n), (block,) * 3)
+        st, ss = np.zeros(nb), np.zeros((nb, 1))
+        for b in range(nb):
+            st[b] = rng.log_uniform(0.1, 50.0)
+            ss[b, 0] = rng.uniform(0.0, 0.95) * st[b]
+        for r in range(n_rhs):
+            qb = np.array([rng.uniform(0.0, 1.0) for _ in range(nb)])
+            specs.append(ProblemSpec(f"batch_instance_{i}_rhs_{r}", 3, 1, (n, n, n),
+                                     (1.0 / n,) * 3, quad, st, ss, mat, qb[mat], ws,
+                                     meta={"seed": seed, "instance": i, "rhs": r,
+                                           "block": block}))
+    return specs
+
+
 def thick_3d(n=32, block=4, seed=4, sn=8, ws=4.0 * np.pi, quad=None):
     quad = quad or Quadrature(LS, 3, sn=sn, L_max=0, weight_sum=ws)
     mat, nb = _block_map((n, n, n), (block,) * 3)
