@@ -39,6 +39,8 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--tune-reps", type=int, default=3)
     ap.add_argument("--skip-sequential", action="store_true")
+    ap.add_argument("--shape", default=None,
+                    help="waves,dir_groups: fixed shape instead of the autotuner")
     ap.add_argument("--out", default=None)
     a = ap.parse_args(argv)
     sizes = tuple(int(s) for s in a.sizes.split(","))
@@ -69,13 +71,18 @@ def main(argv=None):
             f"(measured in {time.perf_counter() - t0:.1f} s)")
 
     t0 = time.perf_counter()
-    batch = ou.Batch(ops, waves=4, dir_groups=1)
+    fixed = tuple(int(v) for v in a.shape.split(",")) if a.shape else (4, 1)
+    batch = ou.Batch(ops, waves=fixed[0], dir_groups=fixed[1])
     log(f"[batch] batch built in {time.perf_counter() - t0:.1f} s: {batch.info()}")
-    table = batch.tune(reps=a.tune_reps)
-    for e in sorted(table, key=lambda e: e["ms"]):
-        log(f"[batch]   waves={e['waves']:2d} dir_groups={e['dir_groups']}  {e['ms']:.2f} ms")
+    table = None
+    if not a.shape:
+        table = batch.tune(reps=a.tune_reps)
+        for e in sorted(table, key=lambda e: e["ms"]):
+            log(f"[batch]   waves={e['waves']:2d} dir_groups={e['dir_groups']}  "
+                f"{e['ms']:.2f} ms")
     info = batch.info()
-    log(f"[batch] autotuned shape: waves={info['waves']} dir_groups={info['dir_groups']}")
+    log(f"[batch] shape: waves={info['waves']} dir_groups={info['dir_groups']} "
+        f"({'fixed' if a.shape else 'autotuned'})")
 
     graph_ms = batch.time_apply(xs, ys, a.warmup, a.steps)
     log(f"[batch] graph replay: {graph_ms:.2f} ms per batched Apply")

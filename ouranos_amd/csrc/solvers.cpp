@@ -89,10 +89,11 @@ SolveResult solve_gmres(Operator &op, const double *d_b, double *d_x, int restar
   const long n = op.DomainSize();
   const int m = restart > 0 ? restart : 30;
   if (m > 31) fail(OU_ERR_ARG, "restart length must be <= 31");
-  DevBuf V(size_t(n) * (m + 1)), w(n), hd(size_t(m + 1));
+  DevBuf V(size_t(n) * (m + 1)), hd(size_t(m + 1));
   std::unique_ptr<DevBuf> Z;
   if (prec) Z.reset(new DevBuf(size_t(n) * m));
   Dotter dt(m + 1);
+  int reorth = 0;  // Arnoldi steps that needed the second Gram-Schmidt pass
   std::vector<double> H(size_t(m + 1) * m, 0.0), cs(m), sn(m), g(m + 1), y(m), h(m + 1);
   SolveResult res;
   const double bnorm = dt.norm(n, d_b, s);
@@ -109,7 +110,9 @@ SolveResult solve_gmres(Operator &op, const double *d_b, double *d_x, int restar
     vec_lincomb(n, 1.0, d_b, -1.0, V.p, V.p, s);
     const double beta = dt.norm(n, V.p, s);
     res.rel_residual = beta / bnorm;
-    if (verbose) progress("GMRES restart at %d: true residual %.3e\n", total, res.rel_residual);
+    if (verbose)
+      progress("GMRES restart at %d: true residual %.3e (%d reorthogonalised steps)\n", total,
+               res.rel_residual, reorth);
     if (res.rel_residual < tol) {
       res.converged = true;
       break;
@@ -121,31 +124,42 @@ SolveResult solve_gmres(Operator &op, const double *d_b, double *d_x, int restar
     int j = 0;
     for (; j < m && total < max_it; ++j) {
       double *vj = V.p + size_t(j) * n;
+      // The new Arnoldi vector is built in place in the next basis slot.
+      double *wj = V.p + size_t(j + 1) * n;
       if (prec) {
         double *zj = Z->p + size_t(j) * n;
         prec->apply(vj, zj, s);
-        op.Apply(zj, w.p, s);
-        if (observer) observer->observe(total, zj, w.p, s);
+        op.Apply(zj, wj, s);
+        if (observer) observer->observe(total, zj, wj, s);
       } else {
-        op.Apply(vj, w.p, s);
-        if (observer) observer->observe(total, vj, w.p, s);
+        op.Apply(vj, wj, s);
+        if (observer) observer->observe(total, vj, wj, s);
       }
       ++total;
-      // Classical Gram-Schmidt, applied twice (FP64).
-      for (int i = 0; i <= j; ++i) h[i] = 0.0;
-      for (int pass = 0; pass < 2; ++pass) {
-        const std::vector<double> &hh = dt.dots(n, j + 1, V.p, n, w.p, s);
-        for (int i = 0; i <= j; ++i) h[i] += hh[i];
+      // Classical Gram-Schmidt in FP64 with the DGKS test: one pass, whose
+      // dot products also give |w|^2, and a second pass only when the norm
+      // fell below 1/sqrt(2) of |w| (Daniel, Gragg, Kaufman and Stewart,
+      // Math. Comp. 30 (1976)).
+      double hn = 0.0;
+      {
+        const std::vector<double> &hh = dt.dots(n, j + 2, V.p, n, wj, s);
+        const double wnorm = std::sqrt(std::max(hh[j + 1], 0.0));
+        for (int i = 0; i <= j; ++i) h[i] = hh[i];
         OU_HIP(hipMemcpy(hd.p, hh.data(), size_t(j + 1) * 8, hipMemcpyHostToDevice));
-        vec_multi_axpy(n, j + 1, V.p, n, hd.p, 1.0, w.p, s);
+        vec_multi_axpy(n, j + 1, V.p, n, hd.p, 1.0, wj, s);
+        hn = dt.norm(n, wj, s);
+        if (hn < 0.7071067811865476 * wnorm) {
+          const std::vector<double> &h2 = dt.dots(n, j + 1, V.p, n, wj, s);
+          for (int i = 0; i <= j; ++i) h[i] += h2[i];
+          OU_HIP(hipMemcpy(hd.p, h2.data(), size_t(j + 1) * 8, hipMemcpyHostToDevice));
+          vec_multi_axpy(n, j + 1, V.p, n, hd.p, 1.0, wj, s);
+          hn = dt.norm(n, wj, s);
+          ++reorth;
+        }
       }
-      const double hn = dt.norm(n, w.p, s);
       for (int i = 0; i <= j; ++i) H[size_t(i) * m + j] = h[i];
       H[size_t(j + 1) * m + j] = hn;
-      if (hn > 0.0) {
-        vec_copy(n, w.p, V.p + size_t(j + 1) * n, s);
-        vec_scale(n, 1.0 / hn, V.p + size_t(j + 1) * n, s);
-      }
+      if (hn > 0.0) vec_scale(n, 1.0 / hn, wj, s);
       for (int i = 0; i < j; ++i) {
         const double a = H[size_t(i) * m + j], b = H[size_t(i + 1) * m + j];
         H[size_t(i) * m + j] = cs[i] * a + sn[i] * b;

@@ -577,11 +577,11 @@ void launch_build_inverses(Plan &pl) {
                        static_cast<const float *>(pl.d_ainv), pl.d_ainv_mfma);
   OU_HIP(hipGetLastError());
   OU_HIP(hipStreamSynchronize(pl.stream));
-  hipFree(dM);
-  hipFree(dG);
-  hipFree(dE);
-  hipFree(dom);
-  hipFree(dst);
+  (void)hipFree(dM);
+  (void)hipFree(dG);
+  (void)hipFree(dE);
+  (void)hipFree(dom);
+  (void)hipFree(dst);
 }
 
 bool sweep_supported(int dim, int p, int mm, int precision, int waves, int max_lds) {

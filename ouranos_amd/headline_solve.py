@@ -37,10 +37,12 @@ def main(argv=None):
     ap.add_argument("--workload", default="headline_3d")
     ap.add_argument("--n", type=int, default=0, help="mesh size override (cells per axis)")
     ap.add_argument("--tol", type=float, default=1e-8)
-    ap.add_argument("--inner-tol", type=float, default=1e-3)
+    ap.add_argument("--inner-tol", type=float, default=3e-5)
     ap.add_argument("--max-it", type=int, default=3000)
     ap.add_argument("--restart", type=int, default=30)
     ap.add_argument("--weights", default=WEIGHTS)
+    ap.add_argument("--grid", type=int, default=0,
+                    help="coarse grid of the network (default: the one it was trained on)")
     ap.add_argument("--skip", default="", help="comma list of plain,refined,learned")
     ap.add_argument("--reference", default=None,
                     help="saved plain FP64 solution (.npy) to compare against")
@@ -77,7 +79,8 @@ def main(argv=None):
 
     net = None
     if "learned" not in skip:
-        net = Lm.load(a.weights)
+        net = Lm.load(a.weights, grid=a.grid or None)
+        log(f"[solve] network on a {net.grid}^3 coarse grid")
     for name, n in (("learned", net), ("refined", None)):
         if name in skip:
             continue

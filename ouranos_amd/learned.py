@@ -230,10 +230,12 @@ def save(net, path):
         json.dump({"grid": net.grid, "mode": net.mode, "n_params": net.n_params}, f)
 
 
-def load(path, max_batch=1):
+def load(path, max_batch=1, grid=None):
+    """Loads saved weights.  The network is fully convolutional, so `grid`
+    may override the coarse grid it was trained on."""
     with open(os.path.splitext(path)[0] + ".json") as f:
         meta = json.load(f)
-    net = ou.UNet(grid=meta["grid"], max_batch=max_batch)
+    net = ou.UNet(grid=grid or meta["grid"], max_batch=max_batch)
     net.mode = meta["mode"]
     net.set_params(np.load(path, allow_pickle=False))
     return net
