@@ -164,7 +164,10 @@ Plan *create_plan(const ou_problem_desc *desc) {
       OU_HIP(hipDeviceGetAttribute(&p.max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
     }
     p.use_mfma = p.precision == OU_FP32 && p.N == 27;
-    p.waves = p.use_mfma ? 12 : 4;
+    // Waves per workgroup: 12 for the matrix-core Q2 solve; 8 for the FP64 Q2
+    // solve, whose inverse rows stream through scalar loads (more waves hide
+    // their latency: 395 ms against 517 ms with 4 on the headline sweep).
+    p.waves = p.use_mfma ? 12 : (p.dim == 3 && p.p == 2 && p.precision == OU_FP64 ? 8 : 4);
     if (const char *env = std::getenv("OURANOS_SWEEP_WAVES")) p.waves = std::atoi(env);
     if (!sweep_supported(p.dim, p.p, p.mm, p.precision, p.waves, p.max_lds)) p.waves = 4;
     if (!sweep_supported(p.dim, p.p, p.mm, p.precision, p.waves, p.max_lds))
@@ -243,14 +246,38 @@ Plan *create_plan(const ou_problem_desc *desc) {
         std::vector<long> &b = bucket[l];
         std::stable_sort(b.begin(), b.end(),
                          [&](long a, long c) { return p.mat[a] < p.mat[c]; });
-        size_t s = 0;
-        while (s < b.size()) {
+        // Half chunks: runs of up to 32 cells of one material.  Consecutive
+        // halves pair into chunks; with MFMA cell solves the two halves of a
+        // chunk may carry different materials, otherwise a chunk ends where
+        // its material does.
+        struct Half {
+          int material;
+          size_t begin, count;
+        };
+        std::vector<Half> halves;
+        for (size_t s = 0; s < b.size();) {
           const int m = p.mat[b[s]];
           size_t e = s;
-          while (e < b.size() && e - s < 64 && p.mat[b[e]] == m) ++e;
-          per_level[l].push_back(Chunk{o, m, int(e - s), int(cells.size())});
-          for (size_t q = s; q < e; ++q) cells.push_back(int(b[q]));
+          while (e < b.size() && p.mat[b[e]] == m) ++e;
+          for (size_t q = s; q < e; q += 32) halves.push_back({m, q, std::min<size_t>(32, e - q)});
           s = e;
+        }
+        for (size_t h = 0; h < halves.size();) {
+          const Half &lo = halves[h];
+          Chunk c{o, lo.material, lo.material, int(lo.count), 0, int(cells.size())};
+          for (size_t q = 0; q < lo.count; ++q) cells.push_back(int(b[lo.begin + q]));
+          ++h;
+          // a partial first half only pairs with another material on the
+          // matrix-core path; a full first half pairs with the next half of
+          // its own material on every path
+          if (h < halves.size() && (p.use_mfma || (lo.count == 32 && halves[h].material == lo.material))) {
+            const Half &hi = halves[h];
+            c.material_hi = hi.material;
+            c.count_hi = int(hi.count);
+            for (size_t q = 0; q < hi.count; ++q) cells.push_back(int(b[hi.begin + q]));
+            ++h;
+          }
+          per_level[l].push_back(c);
         }
       }
     }

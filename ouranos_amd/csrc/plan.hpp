@@ -12,12 +12,18 @@ namespace ou {
 
 constexpr int kMaxDirGroups = 8;
 
-// One chunk: up to 64 cells of one sweep level of one octant, all with the
-// same material (so every lane of a wave uses the same inverse matrix).
+// One chunk: up to 64 cells of one sweep level of one octant.  Lanes 0-31
+// (count_lo cells, material `material`) and lanes 32-63 (count_hi cells,
+// material `material_hi`) each use one inverse matrix; the two halves differ
+// only in plans whose cell solves run on the matrix cores (one 32-column MFMA
+// block per half).  The chunk's cells are cell_offset .. + count_lo + count_hi
+// of the cell list, lanes 0.. then lanes 32...
 struct Chunk {
   int octant;
   int material;
-  int count;
+  int material_hi;
+  int count_lo;
+  int count_hi;
   int cell_offset;  // into the cell list
 };
 
