@@ -136,6 +136,13 @@ const char *ou_label(const ou_plan *plan);
  * y = x - D2M L^-1 M2D S(x) on host vectors (copied in and out). */
 int ou_apply(ou_plan *plan, const double *x, double *y);
 
+/* Apply to n_rhs columns at once, x and y host arrays [rhs][m][dof] (the
+ * column-major layout of a multi-column Epetra_MultiVector).  The reference
+ * Apply reads column 0 only (src/RadiativeTransfer.cc:784-806); every column
+ * here gets the operator.  Columns are applied in order on the plan's stream,
+ * each equal to its own ou_apply result bit-for-bit. */
+int ou_apply_multi(ou_plan *plan, const double *x, double *y, int n_rhs);
+
 /* Apply on device vectors of ou_domain_size doubles, enqueued on `stream`
  * (a hipStream_t, or NULL for the plan's stream).  No host synchronisation. */
 int ou_apply_device(ou_plan *plan, const double *d_x, double *d_y, void *stream);

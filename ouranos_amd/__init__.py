@@ -335,7 +335,15 @@ class RadiativeTransfer:
         return int(lib().ou_range_size(self._plan))
 
     def Apply(self, x):
+        """y = x - D2M L^-1 M2D S(x).  x is one vector of domain_size() entries
+        or an array [n_rhs][domain_size()] of columns (each column applied)."""
         x = np.ascontiguousarray(x, dtype=np.float64)
+        if x.ndim == 2:
+            if x.shape[1] != self.n:
+                raise OuError(-2, f"x has columns of {x.shape[1]} entries, operator expects {self.n}")
+            y = np.empty_like(x)
+            _check(lib().ou_apply_multi(self._plan, _d(x), _d(y), x.shape[0]))
+            return y
         if x.size != self.n:
             raise OuError(-2, f"x has {x.size} entries, operator expects {self.n}")
         y = np.empty(self.n)

@@ -35,6 +35,7 @@ def main(argv=None):
     ap.add_argument("--rhs", type=int, default=4)
     ap.add_argument("--sizes", default="16,24,32,48,64")
     ap.add_argument("--seed", type=int, default=3)
+    ap.add_argument("--block", type=int, default=8, help="material block size (cells per axis)")
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--tune-reps", type=int, default=3)
@@ -45,7 +46,8 @@ def main(argv=None):
     a = ap.parse_args(argv)
     sizes = tuple(int(s) for s in a.sizes.split(","))
 
-    specs = P.batch_instances(n_instances=a.instances, n_rhs=a.rhs, sizes=sizes, seed=a.seed)
+    specs = P.batch_instances(n_instances=a.instances, n_rhs=a.rhs, sizes=sizes, seed=a.seed,
+                              block=a.block)
     cells = sum(s.n_cells for s in specs)
     updates = sum(s.updates_per_sweep() for s in specs)
     alg_bytes = sum(s.algorithmic_bytes_per_sweep() for s in specs)
@@ -108,7 +110,7 @@ def main(argv=None):
     out = {
         "workload": "batch_instances",
         "members": len(specs), "instances": a.instances, "rhs_per_instance": a.rhs,
-        "sizes": list(sizes), "cells": cells, "directions": specs[0].quad.n_dir,
+        "sizes": list(sizes), "block": a.block, "cells": cells, "directions": specs[0].quad.n_dir,
         "precision": "fp32 sweep, fp64 moments",
         "shape": {"waves": info["waves"], "dir_groups": info["dir_groups"]},
         "levels_per_batch": info["n_levels"], "chunks_per_batch": info["n_chunks"],

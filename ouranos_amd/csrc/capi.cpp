@@ -215,6 +215,24 @@ int ou_apply(ou_plan *plan, const double *x, double *y) {
   });
 }
 
+int ou_apply_multi(ou_plan *plan, const double *x, double *y, int n_rhs) {
+  return guarded([&] {
+    ou::Plan &pl = plan_of(plan);
+    if (!x || !y) ou::fail(OU_ERR_ARG, "null vector");
+    if (n_rhs < 1) ou::fail(OU_ERR_ARG, "n_rhs must be positive");
+    const size_t n = size_t(pl.vec_size), bytes = n * 8;
+    check_finite_host(x, long(n) * n_rhs);
+    ensure_staging(pl);
+    for (int r = 0; r < n_rhs; ++r) {
+      OU_HIP(hipMemcpyAsync(pl.d_x, x + r * n, bytes, hipMemcpyHostToDevice, pl.stream));
+      plan->rt->Apply(pl.d_x, pl.d_y, pl.stream);
+      OU_HIP(hipMemcpyAsync(y + r * n, pl.d_y, bytes, hipMemcpyDeviceToHost, pl.stream));
+    }
+    OU_HIP(hipStreamSynchronize(pl.stream));
+    return int(OU_OK);
+  });
+}
+
 int ou_apply_device(ou_plan *plan, const double *d_x, double *d_y, void *stream) {
   return guarded([&] {
     plan_of(plan);

@@ -104,10 +104,17 @@ SolveResult solve_gmres(Operator &op, const double *d_b, double *d_x, int restar
     return res;
   }
   int total = 0;
+  // A zero initial guess needs no operator application for its residual.
+  bool x_is_zero = dt.norm(n, d_x, s) == 0.0;
   while (true) {
     // r = b - A x
-    op.Apply(d_x, V.p, s);
-    vec_lincomb(n, 1.0, d_b, -1.0, V.p, V.p, s);
+    if (x_is_zero) {
+      vec_copy(n, d_b, V.p, s);
+      x_is_zero = false;
+    } else {
+      op.Apply(d_x, V.p, s);
+      vec_lincomb(n, 1.0, d_b, -1.0, V.p, V.p, s);
+    }
     const double beta = dt.norm(n, V.p, s);
     res.rel_residual = beta / bnorm;
     if (verbose)

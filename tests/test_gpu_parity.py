@@ -171,6 +171,20 @@ def test_operator_surface_and_errors():
     assert e.value.code == -6
 
 
+def test_multi_column_apply_equals_columns():
+    """ou_apply_multi: every column of [n_rhs][size] gets the operator, each
+    equal to its own single Apply bit-for-bit."""
+    spec = P.hetero_2d(n=8, block=4, n_polar=4, n_azim=8, L=1)
+    op = spec.operator(ou.FP32)
+    X = np.stack([P.seeded_vector(spec.size, 40 + r) for r in range(3)])
+    Y = op.Apply(X)
+    assert Y.shape == X.shape
+    for r in range(3):
+        assert np.array_equal(Y[r], op.Apply(X[r]))
+    with pytest.raises(ou.OuError):
+        op.Apply(np.zeros((2, spec.size - 1)))
+
+
 def test_apply_properties_at_full_2d_size():
     """256x256 Q2, 256 directions, L=3: linearity and sigma_s = 0 => identity."""
     spec = P.hetero_2d()
