@@ -65,7 +65,7 @@ bool finite_all(const double *p, size_t n) {
 }  // namespace
 
 Plan::~Plan() {
-  void *bufs[] = {d_ainv_mfma, d_ainv,     d_S,        d_S0,       d_slot_m2d, d_slot_c,
+  void *bufs[] = {d_ainv_mfma64, d_ainv_mfma, d_ainv, d_S, d_S0, d_slot_m2d, d_slot_c,
                   d_slot_d2m,  d_partial,  d_trace[0], d_trace[1], d_trace[2], d_oct_dirs,
                   d_oct_off,   d_chunks,   d_cells,    d_mat,      d_mom_l,    d_sigma_s,
                   d_source,    d_mass,     d_rowsum,   d_inflow,   d_x,        d_y,
@@ -164,10 +164,10 @@ Plan *create_plan(const ou_problem_desc *desc) {
       OU_HIP(hipDeviceGetAttribute(&p.max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
     }
     p.use_mfma = p.precision == OU_FP32 && p.N == 27;
-    // Waves per workgroup: 12 for the matrix-core Q2 solve; 8 for the FP64 Q2
-    // solve, whose inverse rows stream through scalar loads (more waves hide
-    // their latency: 395 ms against 517 ms with 4 on the headline sweep).
-    p.waves = p.use_mfma ? 12 : (p.dim == 3 && p.p == 2 && p.precision == OU_FP64 ? 8 : 4);
+    p.use_mfma64 = p.precision == OU_FP64 && p.N == 27;
+    // Waves per workgroup: 12 for the FP32 matrix-core Q2 solve, 4 elsewhere
+    // (the FP64 matrix-core Q2 solve spills registers beyond 4).
+    p.waves = p.use_mfma ? 12 : 4;
     if (const char *env = std::getenv("OURANOS_SWEEP_WAVES")) p.waves = std::atoi(env);
     if (!sweep_supported(p.dim, p.p, p.mm, p.precision, p.waves, p.max_lds)) p.waves = 4;
     if (!sweep_supported(p.dim, p.p, p.mm, p.precision, p.waves, p.max_lds))
@@ -361,6 +361,8 @@ Plan *create_plan(const ou_problem_desc *desc) {
     p.d_ainv = dev_alloc(size_t(p.n_mat) * p.n_cls * p.N * p.N * rs);
     if (p.use_mfma)
       p.d_ainv_mfma = static_cast<float *>(dev_alloc(size_t(p.n_mat) * p.n_cls * 64 * 16 * 4));
+    if (p.use_mfma64)
+      p.d_ainv_mfma64 = static_cast<double *>(dev_alloc(size_t(p.n_mat) * p.n_cls * 64 * 14 * 8));
     p.d_S = dev_alloc(size_t(p.vec_size) * rs);
     OU_HIP(hipMemset(p.d_S, 0, size_t(p.vec_size) * rs));
     p.d_S0 = dev_alloc(size_t(p.n_dofs) * rs);

@@ -39,6 +39,7 @@ struct SweepView {
   const int *slot_cls;     // [slot] class of direction oct_dirs[slot]
   const void *ainv;        // [mat][cls][N][N] in the sweep precision (canonical frame)
   const float *ainv_mfma;  // [mat][cls][64 lanes][16]: MFMA A-operand layout (FP32, N=27)
+  const double *ainv_mfma64;  // [mat][cls][64 lanes][14]: FP64 MFMA A-operand layout (N=27)
   const void *S;           // [m][dof] scattering source (sweep precision)
   const void *S0;          // moment-0 override for the source sweep, or null
   const int *oct_dirs;     // directions grouped by octant ("slots")
@@ -94,6 +95,8 @@ struct Plan {
   void *d_ainv = nullptr, *d_S = nullptr, *d_S0 = nullptr;
   float *d_ainv_mfma = nullptr;
   bool use_mfma = false;
+  double *d_ainv_mfma64 = nullptr;
+  bool use_mfma64 = false;
   int max_lds = 65536;
   void *d_slot_m2d = nullptr, *d_slot_c = nullptr;
   double *d_slot_d2m = nullptr, *d_partial = nullptr;

@@ -258,6 +258,22 @@ __global__ __launch_bounds__(64) void k_ainv_to_mfma(const float *__restrict__ a
   }
 }
 
+// The same for FP64 Q2 and v_mfma_f64_16x16x4f64: out[md][lane][7 mt + ks] =
+// A[16 mt + lane%16][4 ks + lane/16] (zero padded to 32 x 28), the A operand of
+// row tile mt and k-step ks of solve_mfma27_f64.
+__global__ __launch_bounds__(64) void k_ainv_to_mfma64(const double *__restrict__ ainv,
+                                                       double *__restrict__ out) {
+  const int lane = threadIdx.x;
+  const long md = blockIdx.x;
+  const double *A = ainv + md * 27 * 27;
+  double *o = out + (md * 64 + lane) * 14;
+  for (int mt = 0; mt < 2; ++mt)
+    for (int ks = 0; ks < 7; ++ks) {
+      const int row = 16 * mt + (lane & 15), col = 4 * ks + (lane >> 4);
+      o[7 * mt + ks] = (row < 27 && col < 27) ? A[row * 27 + col] : 0.0;
+    }
+}
+
 // y = x - sum_o partial_o (Apply) or y += sum_o partial_o (sweep), summed
 // over the n_parts = octants x direction groups partial moment vectors.
 __device__ __forceinline__ void epilogue_one(long gid, int n_parts, long stride,
@@ -517,6 +533,7 @@ SweepView Plan::view() const {
   v.slot_cls = d_slot_cls;
   v.ainv = d_ainv;
   v.ainv_mfma = d_ainv_mfma;
+  v.ainv_mfma64 = d_ainv_mfma64;
   v.S = d_S;
   v.S0 = nullptr;
   v.slot_m2d = d_slot_m2d;
@@ -575,6 +592,9 @@ void launch_build_inverses(Plan &pl) {
   if (pl.use_mfma)
     hipLaunchKernelGGL(k_ainv_to_mfma, dim3(blocks), dim3(64), 0, pl.stream,
                        static_cast<const float *>(pl.d_ainv), pl.d_ainv_mfma);
+  if (pl.use_mfma64)
+    hipLaunchKernelGGL(k_ainv_to_mfma64, dim3(blocks), dim3(64), 0, pl.stream,
+                       static_cast<const double *>(pl.d_ainv), pl.d_ainv_mfma64);
   OU_HIP(hipGetLastError());
   OU_HIP(hipStreamSynchronize(pl.stream));
   (void)hipFree(dM);
