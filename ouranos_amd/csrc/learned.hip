@@ -327,8 +327,12 @@ RefinedResult solve_refined(RadiativeTransfer &rt64, RadiativeTransfer &rt32,
     }
     if (res.total.iterations >= max_it) break;
     vec_fill(n, 0.0, d, s);
-    const SolveResult in = solve_gmres(rt32, r, d, restart, inner_tol,
-                                       max_it - res.total.iterations, prec, verbose, s);
+    // An inner solve that stagnates above inner_tol (the FP32 operator's own
+    // accuracy can stop it) hands over to a new FP64 residual after four
+    // restart cycles.
+    const int inner_max = std::min(max_it - res.total.iterations, 4 * std::max(restart, 1));
+    const SolveResult in =
+        solve_gmres(rt32, r, d, restart, inner_tol, inner_max, prec, verbose, s);
     res.total.iterations += in.iterations;
     vec_axpby(n, 1.0, d, 1.0, d_phi, s);
     if (in.iterations == 0) break;

@@ -36,6 +36,8 @@ def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--workload", default="headline_3d")
     ap.add_argument("--n", type=int, default=0, help="mesh size override (cells per axis)")
+    ap.add_argument("--block", type=int, default=0, help="material block size override")
+    ap.add_argument("--seed", type=int, default=-1, help="material seed override")
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--inner-tol", type=float, default=3e-5)
     ap.add_argument("--max-it", type=int, default=3000)
@@ -53,7 +55,14 @@ def main(argv=None):
     skip = set(s for s in a.skip.split(",") if s)
 
     builder = getattr(P, a.workload)
-    spec = builder(n=a.n) if a.n else builder()
+    kw = {}
+    if a.n:
+        kw["n"] = a.n
+    if a.block:
+        kw["block"] = a.block
+    if a.seed >= 0:
+        kw["seed"] = a.seed
+    spec = builder(**kw)
     log(f"[solve] {spec.name}: n={spec.n}, p={spec.p}, {spec.quad.n_dir} directions, "
         f"{spec.quad.n_mom} moments, {spec.size} unknowns")
     t0 = time.perf_counter()
