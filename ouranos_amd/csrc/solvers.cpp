@@ -37,6 +37,15 @@ struct Dotter {
   const std::vector<double> &dots(long n, int m, const double *V, long ldv, const double *w,
                                   hipStream_t s) {
     vec_multi_dot(n, m, V, ldv, w, scratch.p, out.p, s);
+    return fetch(m, s);
+  }
+  // w -= V h (h on the device); returns <V_i, w> (i < m, when `dots`) then <w, w>.
+  const std::vector<double> &axpy_dots(long n, int m, const double *V, long ldv,
+                                       const double *h, double *w, bool dots, hipStream_t s) {
+    vec_axpy_dots(n, m, V, ldv, h, w, dots, scratch.p, out.p, s);
+    return fetch(dots ? m + 1 : 1, s);
+  }
+  const std::vector<double> &fetch(int m, hipStream_t s) {
     OU_HIP(hipMemcpyAsync(host.data(), out.p, size_t(m) * 8, hipMemcpyDeviceToHost, s));
     OU_HIP(hipStreamSynchronize(s));
     return host;
@@ -143,24 +152,25 @@ SolveResult solve_gmres(Operator &op, const double *d_b, double *d_x, int restar
         if (observer) observer->observe(total, vj, wj, s);
       }
       ++total;
-      // Classical Gram-Schmidt in FP64 with the DGKS test: one pass, whose
-      // dot products also give |w|^2, and a second pass only when the norm
-      // fell below 1/sqrt(2) of |w| (Daniel, Gragg, Kaufman and Stewart,
-      // Math. Comp. 30 (1976)).
+      // Classical Gram-Schmidt in FP64 with the DGKS test: a second pass only
+      // when the norm fell below 1/sqrt(2) of |w| (Daniel, Gragg, Kaufman and
+      // Stewart, Math. Comp. 30 (1976)).  Three sweeps over the basis: the
+      // first dot products (with |w|^2), the first projection fused with the
+      // second pass's dot products and the new norm, and the second
+      // projection fused with the final norm.
       double hn = 0.0;
       {
         const std::vector<double> &hh = dt.dots(n, j + 2, V.p, n, wj, s);
         const double wnorm = std::sqrt(std::max(hh[j + 1], 0.0));
         for (int i = 0; i <= j; ++i) h[i] = hh[i];
         OU_HIP(hipMemcpy(hd.p, hh.data(), size_t(j + 1) * 8, hipMemcpyHostToDevice));
-        vec_multi_axpy(n, j + 1, V.p, n, hd.p, 1.0, wj, s);
-        hn = dt.norm(n, wj, s);
+        const std::vector<double> &h2 = dt.axpy_dots(n, j + 1, V.p, n, hd.p, wj, true, s);
+        hn = std::sqrt(std::max(h2[j + 1], 0.0));
         if (hn < 0.7071067811865476 * wnorm) {
-          const std::vector<double> &h2 = dt.dots(n, j + 1, V.p, n, wj, s);
           for (int i = 0; i <= j; ++i) h[i] += h2[i];
           OU_HIP(hipMemcpy(hd.p, h2.data(), size_t(j + 1) * 8, hipMemcpyHostToDevice));
-          vec_multi_axpy(n, j + 1, V.p, n, hd.p, 1.0, wj, s);
-          hn = dt.norm(n, wj, s);
+          const std::vector<double> &h3 = dt.axpy_dots(n, j + 1, V.p, n, hd.p, wj, false, s);
+          hn = std::sqrt(std::max(h3[0], 0.0));
           ++reorth;
         }
       }

@@ -136,6 +136,83 @@ __global__ __launch_bounds__(kBlock) void k_multi_axpy(long n, int m, const doub
   }
 }
 
+// w -= sum_{i<m} h[i] V_i, then, in the same pass, the partial sums of
+// <V_i, w> for i < ndots (ndots is 0 or m) and of <w, w> (slot ndots) per
+// block: the projection of one Gram-Schmidt pass fused with the dot products
+// of the next and the norm of the result.  A step's m <= 16 vector values
+// stay in registers between the update and the dot products.
+template <int M, int VEC>
+__global__ __launch_bounds__(kBlock) void k_axpy_dots(long n, int m, const double *V, long ldv,
+                                                      const double *h, double *w, int ndots,
+                                                      double *partial) {
+  __shared__ double hs[M];
+  __shared__ double red[kWaves][M + 1];
+  if (threadIdx.x < m) hs[threadIdx.x] = h[threadIdx.x];
+  __syncthreads();
+  double acc[M + 1];
+#pragma unroll
+  for (int i = 0; i <= M; ++i) acc[i] = 0.0;
+  const long stride = long(gridDim.x) * kBlock;
+  if (VEC == 2) {
+    double2 *w2 = reinterpret_cast<double2 *>(w);
+    for (long k = long(blockIdx.x) * kBlock + threadIdx.x; k < n / 2; k += stride) {
+      double2 vv[M];
+      double sx = 0.0, sy = 0.0;
+#pragma unroll
+      for (int i = 0; i < M; ++i)
+        if (i < m) {
+          vv[i] = reinterpret_cast<const double2 *>(V + i * ldv)[k];
+          sx += hs[i] * vv[i].x;
+          sy += hs[i] * vv[i].y;
+        }
+      double2 wk = w2[k];
+      wk.x -= sx;
+      wk.y -= sy;
+      w2[k] = wk;
+#pragma unroll
+      for (int i = 0; i < M; ++i)
+        if (i < ndots) acc[i] += vv[i].x * wk.x + vv[i].y * wk.y;
+      acc[M] += wk.x * wk.x + wk.y * wk.y;
+    }
+    if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) {
+      double sum = 0.0;
+      for (int i = 0; i < m; ++i) sum += hs[i] * V[i * ldv + n - 1];
+      const double wk = w[n - 1] - sum;
+      w[n - 1] = wk;
+      for (int i = 0; i < M; ++i)
+        if (i < ndots) acc[i] += V[i * ldv + n - 1] * wk;
+      acc[M] += wk * wk;
+    }
+  } else {
+    for (long k = long(blockIdx.x) * kBlock + threadIdx.x; k < n; k += stride) {
+      double sum = 0.0;
+#pragma unroll
+      for (int i = 0; i < M; ++i)
+        if (i < m) sum += hs[i] * V[i * ldv + k];
+      const double wk = w[k] - sum;
+      w[k] = wk;
+#pragma unroll
+      for (int i = 0; i < M; ++i)
+        if (i < ndots) acc[i] += V[i * ldv + k] * wk;
+      acc[M] += wk * wk;
+    }
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int i = 0; i <= M; ++i)
+    if (i < ndots || i == M) {
+      const double t = wave_sum(acc[i]);
+      if (lane == 0) red[wave][i] = t;
+    }
+  __syncthreads();
+  if (threadIdx.x <= ndots) {
+    const int i = threadIdx.x < ndots ? threadIdx.x : M;
+    double t = 0.0;
+    for (int q = 0; q < kWaves; ++q) t += red[q][i];
+    partial[long(blockIdx.x) * (ndots + 1) + threadIdx.x] = t;
+  }
+}
+
 // Element-wise kernels: z = a x + b y (x, y may be null when unused), in
 // double2 steps when aligned (VEC 2), with a scalar tail for odd n.
 // kind: 0 axpby (y = a x + b y), 1 lincomb (z = a x + b y), 2 scale (x *= a),
@@ -195,6 +272,17 @@ void launch_dot(bool v2, int nb, hipStream_t s, long n, int m, const double *V, 
                        scratch);
 }
 
+template <int M>
+void launch_axpy_dots(bool v2, int nb, hipStream_t s, long n, int m, const double *V, long ldv,
+                      const double *h, double *w, int ndots, double *scratch) {
+  if (v2)
+    hipLaunchKernelGGL((k_axpy_dots<M, 2>), dim3(nb), dim3(kBlock), 0, s, n, m, V, ldv, h, w,
+                       ndots, scratch);
+  else
+    hipLaunchKernelGGL((k_axpy_dots<M, 1>), dim3(nb), dim3(kBlock), 0, s, n, m, V, ldv, h, w,
+                       ndots, scratch);
+}
+
 }  // namespace
 
 long reduce_scratch_doubles(int m) { return long(kMaxBlocks) * (m < 1 ? 1 : m); }
@@ -214,6 +302,34 @@ void vec_multi_dot(long n, int m, const double *V, long ldv, const double *w,
   else
     launch_dot<kMaxDots>(v2, nb, s, n, m, V, ldv, w, scratch);
   hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, s, nb, m, scratch, out);
+  OU_HIP(hipGetLastError());
+}
+
+void vec_axpy_dots(long n, int m, const double *V, long ldv, const double *h, double *w,
+                   bool dots, double *scratch, double *out, hipStream_t s) {
+  if (m < 1 || m > kMaxDots) fail(OU_ERR_ARG, "axpy_dots takes 1 to 32 vectors");
+  const int ndots = dots ? m : 0;
+  if (m > 16) {
+    // Beyond 16 vectors the fused kernel holds too many registers: update,
+    // then one dot-product pass (which also takes the norm when w follows
+    // the m vectors in memory, as the next Arnoldi vector does).
+    vec_multi_axpy(n, m, V, ldv, h, 1.0, w, s);
+    if (dots && w == V + long(m) * ldv && m + 1 <= kMaxDots) {
+      vec_multi_dot(n, m + 1, V, ldv, w, scratch, out, s);
+    } else {
+      if (dots) vec_multi_dot(n, m, V, ldv, w, scratch, out, s);
+      vec_multi_dot(n, 1, w, 0, w, scratch, out + ndots, s);
+    }
+    return;
+  }
+  const bool v2 = aligned16(V) && aligned16(w) && (m == 1 || ldv % 2 == 0);
+  const int nb = blocks_for(v2 ? (n + 1) / 2 : n);
+  if (m <= 8)
+    launch_axpy_dots<8>(v2, nb, s, n, m, V, ldv, h, w, ndots, scratch);
+  else
+    launch_axpy_dots<16>(v2, nb, s, n, m, V, ldv, h, w, ndots, scratch);
+  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, s, nb, ndots + 1, scratch,
+                     out);
   OU_HIP(hipGetLastError());
 }
 

@@ -13,6 +13,12 @@ long reduce_scratch_doubles(int m);
 // `out` is device memory; partial sums use `scratch`.
 void vec_multi_dot(long n, int m, const double *V, long ldv, const double *w,
                    double *scratch, double *out, hipStream_t s);
+// w -= sum_{i<m} h[i] V_i (h on the device, m <= 32), then out[i] =
+// <V_i, w_new> for i < m when `dots`, and out[m or 0] = <w_new, w_new>: one
+// fused pass for m <= 16, an update and one dot-product pass beyond.
+// scratch: reduce_scratch_doubles(m + 1) doubles.
+void vec_axpy_dots(long n, int m, const double *V, long ldv, const double *h, double *w,
+                   bool dots, double *scratch, double *out, hipStream_t s);
 // w -= sum_i h[i] V_i  (h on the device, m <= 64)
 void vec_multi_axpy(long n, int m, const double *V, long ldv, const double *h,
                     double alpha, double *w, hipStream_t s);
