@@ -72,7 +72,7 @@ __global__ __launch_bounds__(kBlock) void k_multi_dot(long n, int m, const doubl
       if (lane == 0) red[wave][i] = s;
     }
   __syncthreads();
-  if (threadIdx.x < m) {
+  if (int(threadIdx.x) < m) {
     double s = 0.0;
     for (int q = 0; q < kWaves; ++q) s += red[q][threadIdx.x];
     partial[long(blockIdx.x) * m + threadIdx.x] = s;
@@ -104,7 +104,7 @@ __global__ __launch_bounds__(kBlock) void k_multi_axpy(long n, int m, const doub
                                                        long ldv, const double *h,
                                                        double alpha, double *w) {
   __shared__ double hs[64];
-  if (threadIdx.x < m) hs[threadIdx.x] = alpha * h[threadIdx.x];
+  if (int(threadIdx.x) < m) hs[threadIdx.x] = alpha * h[threadIdx.x];
   __syncthreads();
   const long stride = long(gridDim.x) * kBlock;
   if (VEC == 2) {
@@ -147,7 +147,7 @@ __global__ __launch_bounds__(kBlock) void k_axpy_dots(long n, int m, const doubl
                                                       double *partial) {
   __shared__ double hs[M];
   __shared__ double red[kWaves][M + 1];
-  if (threadIdx.x < m) hs[threadIdx.x] = h[threadIdx.x];
+  if (int(threadIdx.x) < m) hs[threadIdx.x] = h[threadIdx.x];
   __syncthreads();
   double acc[M + 1];
 #pragma unroll
@@ -205,8 +205,8 @@ __global__ __launch_bounds__(kBlock) void k_axpy_dots(long n, int m, const doubl
       if (lane == 0) red[wave][i] = t;
     }
   __syncthreads();
-  if (threadIdx.x <= ndots) {
-    const int i = threadIdx.x < ndots ? threadIdx.x : M;
+  if (int(threadIdx.x) <= ndots) {
+    const int i = int(threadIdx.x) < ndots ? int(threadIdx.x) : M;
     double t = 0.0;
     for (int q = 0; q < kWaves; ++q) t += red[q][i];
     partial[long(blockIdx.x) * (ndots + 1) + threadIdx.x] = t;
