@@ -267,6 +267,13 @@ int ou_learned_pairs(ou_plan *plan, int grid, const double *e0, int n_steps,
 int ou_learned_pairs_krylov(ou_plan *plan, int grid, int n_steps, int restart, int stride,
                             int max_records, double *r_out, double *e_out);
 
+/* The same from FGMRES(restart) right-preconditioned by `net`: every step's
+ * preconditioned vector z_j comes with A z_j, so (R A z_j, R z_j) is an exact
+ * pair drawn from the residuals the network meets inside FGMRES (training on
+ * the solver's own distribution).  grid must be the network's grid. */
+int ou_learned_pairs_fgmres(ou_plan *plan, ou_unet *net, int grid, int n_steps, int restart,
+                            int stride, int max_records, double *r_out, double *e_out);
+
 /* FGMRES(restart) on the plan's operator, right-preconditioned by the
  * network (opts->solver must be OU_SOLVER_FGMRES; net NULL = unpreconditioned). */
 int ou_solve_learned(ou_plan *plan, ou_unet *net, const ou_solver_opts *opts, double *phi,

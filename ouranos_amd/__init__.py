@@ -136,6 +136,8 @@ def lib():
                                        ctypes.c_ulonglong, _dp, _dp]
         L.ou_learned_pairs_krylov.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                               ctypes.c_int, ctypes.c_int, _dp, _dp]
+        L.ou_learned_pairs_fgmres.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_int,
+                                              ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp]
         L.ou_solve_learned.argtypes = [_vp, _vp, ctypes.POINTER(SolverOpts), _dp,
                                        ctypes.POINTER(SolveInfo)]
         L.ou_solve_refined.argtypes = [_vp, _vp, _vp, ctypes.POINTER(SolverOpts),
@@ -433,13 +435,21 @@ class RadiativeTransfer:
                                        c.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), _d(z)))
         return z
 
-    def learned_pairs_krylov(self, grid, n_steps, stride=1, max_records=64, restart=30):
-        """Coarse pairs (R A v_j, R v_j) of plain GMRES's Arnoldi vectors."""
+    def learned_pairs_krylov(self, grid, n_steps, stride=1, max_records=64, restart=30,
+                             net=None):
+        """Coarse pairs (R A v_j, R v_j) of plain GMRES's Arnoldi vectors, or with
+        `net` the pairs (R A z_j, R z_j) of FGMRES preconditioned by it."""
         P0 = grid ** 3
         r = np.empty((max_records, P0, 8))
         e = np.empty((max_records, P0, 8))
-        got = lib().ou_learned_pairs_krylov(self._plan, int(grid), int(n_steps), int(restart),
-                                            int(stride), int(max_records), _d(r), _d(e))
+        if net is not None:
+            got = lib().ou_learned_pairs_fgmres(self._plan, net._net, int(grid), int(n_steps),
+                                                int(restart), int(stride), int(max_records),
+                                                _d(r), _d(e))
+        else:
+            got = lib().ou_learned_pairs_krylov(self._plan, int(grid), int(n_steps),
+                                                int(restart), int(stride), int(max_records),
+                                                _d(r), _d(e))
         if got < 0:
             _check(got)
         return r[:got], e[:got]

@@ -827,9 +827,15 @@ struct PairRecorder : ou::ArnoldiObserver {
 
 }  // namespace
 
-int ou_learned_pairs_krylov(ou_plan *plan, int grid, int n_steps, int restart, int stride,
-                            int max_records, double *r_out, double *e_out) {
-  return guarded([&] {
+namespace {
+
+// Records (R A z_j, R z_j) of n_steps Arnoldi steps of GMRES(restart) on the
+// plan's source problem; z_j is the Arnoldi vector (prec null) or the
+// preconditioned one (FGMRES with the network).
+int learned_pairs_from_krylov(ou_plan *plan, ou::Preconditioner *prec, int grid, int n_steps,
+                              int restart, int stride, int max_records, double *r_out,
+                              double *e_out) {
+  {
     ou::Plan &pl = plan_of(plan);
     ou::RadiativeTransfer &rt = *plan->rt;
     ou::coarse_factor(pl, grid);
@@ -851,9 +857,31 @@ int ou_learned_pairs_krylov(ou_plan *plan, int grid, int n_steps, int restart, i
     ou::build_rhs(rt, d_b, s);
     ou::vec_fill(n, 0.0, d_x, s);
     PairRecorder rec(pl, grid, stride, max_records, d_c, r_out, e_out);
-    ou::solve_gmres(rt, d_b, d_x, restart, 1e-300, n_steps, nullptr, false, s, &rec);
+    ou::solve_gmres(rt, d_b, d_x, restart, 1e-300, n_steps, prec, false, s, &rec);
     OU_HIP(hipStreamSynchronize(s));
     return rec.count;
+  }
+}
+
+}  // namespace
+
+int ou_learned_pairs_krylov(ou_plan *plan, int grid, int n_steps, int restart, int stride,
+                            int max_records, double *r_out, double *e_out) {
+  return guarded([&] {
+    return learned_pairs_from_krylov(plan, nullptr, grid, n_steps, restart, stride,
+                                     max_records, r_out, e_out);
+  });
+}
+
+int ou_learned_pairs_fgmres(ou_plan *plan, ou_unet *net, int grid, int n_steps, int restart,
+                            int stride, int max_records, double *r_out, double *e_out) {
+  return guarded([&] {
+    plan_of(plan);
+    if (!net || !net->net) ou::fail(OU_ERR_ARG, "null network");
+    if (net->net->grid() != grid) ou::fail(OU_ERR_ARG, "network grid differs from `grid`");
+    ou::LearnedPreconditioner prec(*plan->rt, *net->net);
+    return learned_pairs_from_krylov(plan, &prec, grid, n_steps, restart, stride, max_records,
+                                     r_out, e_out);
   });
 }
 

@@ -85,11 +85,13 @@ KRYLOV_STEPS = 90   # plain GMRES(30) applications per instance
 KRYLOV_STRIDE = 6   # record every 6th Arnoldi pair
 
 
-def pairs_of(op, spec, grid, seed, sources=("si", "krylov")):
+def pairs_of(op, spec, grid, seed, sources=("si", "krylov"), net=None):
     """Features [k][G^3][12] and targets [k][G^3][8] (float32) of one instance.
     Sources: "si" = the error-propagation chains of source iteration from the
     right-hand side and from a random field; "krylov" = the Arnoldi vectors
-    of plain GMRES(30) on the instance's source problem."""
+    of plain GMRES(30) on the instance's source problem; "fgmres" = the
+    preconditioned vectors of FGMRES(30) with `net` itself (the residuals the
+    network meets when it is used)."""
     _, mat = op.coarse_features(grid)
     chunks = []
     if "si" in sources:
@@ -98,6 +100,12 @@ def pairs_of(op, spec, grid, seed, sources=("si", "krylov")):
     if "krylov" in sources:
         chunks.append(op.learned_pairs_krylov(grid, KRYLOV_STEPS, stride=KRYLOV_STRIDE,
                                               max_records=KRYLOV_STEPS // KRYLOV_STRIDE))
+    if "fgmres" in sources:
+        if net is None:
+            raise ValueError("the fgmres source needs the network")
+        chunks.append(op.learned_pairs_krylov(grid, KRYLOV_STEPS, stride=KRYLOV_STRIDE,
+                                              max_records=KRYLOV_STEPS // KRYLOV_STRIDE,
+                                              net=net))
     feats, targets = [], []
     for r, e in chunks:
         for rk, ek in zip(r, e):
@@ -110,14 +118,14 @@ def pairs_of(op, spec, grid, seed, sources=("si", "krylov")):
 
 
 def generate(n_instances, seed0, grid=32, n=32, log=print, sources=("si", "krylov"),
-             family=None):
+             family=None, net=None):
     fam = family or Family("thick")
     feats, targets = [], []
     t0 = time.perf_counter()
     for i in range(n_instances):
         spec = fam.spec(seed0 + i) if family else instance(seed0 + i, n=n, quad=fam.quad)
         op = spec.operator(fam.precision)
-        f, t = pairs_of(op, spec, grid, seed0 + i, sources)
+        f, t = pairs_of(op, spec, grid, seed0 + i, sources, net=net)
         op.close()
         feats += f
         targets += t
@@ -164,8 +172,9 @@ def train_streaming(net, n_instances, seed0, grid, chunk, passes, batch, lr, val
     t0 = time.perf_counter()
     for c in range(n_chunks):
         lo = c * chunk
+        # the fgmres source draws its pairs from the network as trained so far
         F, T = generate(min(chunk, n_instances - lo), seed0 + lo, grid=grid,
-                        log=lambda m: None, sources=sources, family=family)
+                        log=lambda m: None, sources=sources, family=family, net=net)
         if est_steps is None:
             est_steps = n_chunks * passes * (F.shape[0] // batch)
         losses = []
@@ -293,9 +302,6 @@ def main(argv=None):
         KRYLOV_STEPS, KRYLOV_STRIDE = a.krylov_steps, a.krylov_stride
         fam = Family(a.family)
         sources = tuple(a.sources.split(","))
-        log(f"[learned] generating {a.val_instances} validation instances ({a.family})")
-        Fv, Tv = generate(a.val_instances, EVAL_SEED0 + 100_000, grid=a.grid, log=log,
-                          sources=sources, family=fam)
         net = ou.UNet(grid=a.grid, max_batch=a.batch, seed=a.seed)
         net.mode = a.mode
         if a.init:
@@ -303,6 +309,9 @@ def main(argv=None):
             net.mode = init.mode
             net.set_params(init.get_params())
             init.close()
+        log(f"[learned] generating {a.val_instances} validation instances ({a.family})")
+        Fv, Tv = generate(a.val_instances, EVAL_SEED0 + 100_000, grid=a.grid, log=log,
+                          sources=sources, family=fam, net=net)
         v0 = validation_loss(net, Fv, Tv, a.batch)
         log(f"[learned] validation loss before training {v0:.4f}; training on {a.instances} "
             f"instances in chunks of {a.chunk}, {a.passes} passes each")
@@ -322,7 +331,9 @@ def main(argv=None):
                "sources": {"si": "SI error-propagation chains (RHS and random field, steps "
                                  f"{list(RECORD)})",
                            "krylov": f"plain GMRES(30) Arnoldi pairs (every {KRYLOV_STRIDE}th "
-                                     f"of {KRYLOV_STEPS})"},
+                                     f"of {KRYLOV_STEPS})",
+                           "fgmres": f"FGMRES(30) pairs (A z_j, z_j) with the network being "
+                                     f"trained (every {KRYLOV_STRIDE}th of {KRYLOV_STEPS})"},
                "sources_used": list(sources),
                "train_command": "python -m ouranos_amd.learned " + " ".join(argv or sys.argv[1:])}
         if a.eval_instances:

@@ -253,6 +253,29 @@ def test_fgmres_with_silent_network_is_plain_gmres():
     n.close()
 
 
+def test_fgmres_pairs_of_silent_network_are_the_krylov_pairs():
+    """ou_learned_pairs_fgmres records (R A z_j, R z_j) of FGMRES with the
+    network; with a zero head z_j = v_j, so the pairs equal the plain GMRES
+    ones, and a network that predicts something changes them."""
+    spec = _small_thick(n=8)
+    op = spec.operator(ou.FP64)
+    n = ou.UNet(grid=4, max_batch=1, seed=7)
+    p = n.get_params()
+    lay, hw, hb, _ = U.layout()
+    p0 = p.copy()
+    p0[hw:] = 0.0
+    n.set_params(p0)
+    r0, e0 = op.learned_pairs_krylov(4, 12, stride=3, max_records=4)
+    r1, e1 = op.learned_pairs_krylov(4, 12, stride=3, max_records=4, net=n)
+    assert r0.shape == r1.shape == (4, 64, 8)
+    assert np.allclose(r1, r0, rtol=1e-12, atol=1e-14) and np.allclose(e1, e0, rtol=1e-12, atol=1e-14)
+    n.set_params(p)
+    r2, e2 = op.learned_pairs_krylov(4, 12, stride=3, max_records=4, net=n)
+    assert r2.shape == (4, 64, 8)
+    assert not np.allclose(e2, e0)
+    n.close()
+
+
 def test_refinement_reaches_fp64_tolerance_with_fp32_sweeps():
     spec = _small_thick(n=8)
     op64, op32 = spec.operator(ou.FP64), spec.operator(ou.FP32)
