@@ -25,7 +25,7 @@ from ouranos_amd import learned as Lm
 from ouranos_amd import problems as P
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-WEIGHTS = os.path.join(HERE, "weights", "unet_c5.npy")
+WEIGHTS = os.path.join(HERE, "weights", "unet_c5_s4.npy")
 
 
 def log(msg):
