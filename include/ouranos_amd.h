@@ -343,6 +343,19 @@ int ou_batch_apply_device(ou_batch *batch, const double *const *d_x, double *con
 int ou_batch_sweep_device(ou_batch *batch, double *const *d_y, int with_source,
                           void *stream);
 
+/* Source iteration (ouranos.cc:52-130) of every member at once from the
+ * initial guesses d_phi (n_plans device vectors, updated in place):
+ * phi_i <- D L^-1 (M2D S(phi_i) + M q_i/ws + inflow), one captured graph per
+ * iteration for the whole batch, until every member's change
+ * |phi_new - phi| / |phi_new| is below tol or max_it iterations.
+ * iterations[i] = the iteration at which member i first met tol (-1 if never),
+ * change[i] = its last change; *n_iterations = iterations run.  A member that
+ * met tol is not swept again: d_phi[i] returns that iterate.  Each member's
+ * iterates equal its own ou_solve SI iterates bit-for-bit.  verbose != 0
+ * prints the progress every 25 iterations. */
+int ou_batch_solve_si(ou_batch *batch, double *const *d_phi, double tol, int max_it,
+                      int *iterations, double *change, int *n_iterations, int verbose);
+
 /* Re-runs the autotuner (reps timed applications per shape), keeps the
  * fastest shape and writes up to max_entries timings. */
 int ou_batch_tune(ou_batch *batch, int reps, ou_tune_entry *entries, int max_entries,

@@ -155,6 +155,8 @@ def lib():
         L.ou_batch_sweep_device.argtypes = [_vp, ctypes.POINTER(_vp), ctypes.c_int, _vp]
         L.ou_batch_tune.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(TuneEntry), ctypes.c_int,
                                     _ip]
+        L.ou_batch_solve_si.argtypes = [_vp, ctypes.POINTER(_vp), ctypes.c_double, ctypes.c_int,
+                                        _ip, _dp, _ip, ctypes.c_int]
         L.ou_batch_time_apply.argtypes = [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
                                           ctypes.c_int, ctypes.c_int,
                                           ctypes.POINTER(ctypes.c_double)]
@@ -568,6 +570,36 @@ class Batch:
             return out
         finally:
             for b in dx + dy:
+                b.free()
+
+    def solve_si(self, tol=1e-8, max_it=1000, phis=None, verbose=False):
+        """Source iteration of every member at once (zero initial guesses unless
+        `phis` gives host vectors).  Returns (list of solutions, info) with
+        per-member first-convergence iterations (-1: not converged) and last
+        relative changes."""
+        n = len(self.ops)
+        bufs = []
+        try:
+            for i, op in enumerate(self.ops):
+                b = DeviceBuffer(op.n * 8)
+                x = np.zeros(op.n) if phis is None else np.ascontiguousarray(phis[i], np.float64)
+                b.upload(x)
+                bufs.append(b)
+            its = np.zeros(n, dtype=np.int32)
+            ch = np.zeros(n)
+            total = ctypes.c_int()
+            _check(lib().ou_batch_solve_si(self._b, _ptr_table(bufs), float(tol), int(max_it),
+                                           _i(its), _d(ch), ctypes.byref(total),
+                                           int(bool(verbose))))
+            out = []
+            for b, op in zip(bufs, self.ops):
+                y = np.empty(op.n)
+                b.download(y)
+                out.append(y)
+            return out, {"iterations": its.tolist(), "change": ch.tolist(),
+                         "n_iterations": total.value}
+        finally:
+            for b in bufs:
                 b.free()
 
     def tune(self, reps=3):

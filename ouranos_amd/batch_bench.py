@@ -42,6 +42,9 @@ def main(argv=None):
     ap.add_argument("--skip-sequential", action="store_true")
     ap.add_argument("--shape", default=None,
                     help="waves,dir_groups: fixed shape instead of the autotuner")
+    ap.add_argument("--si-tol", type=float, default=0.0,
+                    help="also solve every member by batched source iteration to this change")
+    ap.add_argument("--si-max-it", type=int, default=1000)
     ap.add_argument("--out", default=None)
     a = ap.parse_args(argv)
     sizes = tuple(int(s) for s in a.sizes.split(","))
@@ -106,6 +109,37 @@ def main(argv=None):
         exact &= bool(np.array_equal(y, ops[i].Apply(x)))
     log(f"[batch] members equal their single-plan Apply bit-for-bit: {exact}")
 
+    si = None
+    if a.si_tol > 0:
+        t0 = time.perf_counter()
+        _, sinfo = batch.solve_si(tol=a.si_tol, max_it=a.si_max_it, verbose=True)
+        si_s = time.perf_counter() - t0
+        its = sinfo["iterations"]
+        # the same solves one member at a time (ou_solve SI) for every 8th member
+        # in size order, scaled by 8 to the whole batch
+        order = np.argsort([s.n_cells for s in specs])
+        sample = [int(i) for i in order[::8]]
+        t0 = time.perf_counter()
+        seq_its = []
+        for i in sample:
+            _, info_i = ops[i].solve(solver=ou.SI, tol=a.si_tol, max_it=a.si_max_it,
+                                     raise_on_fail=False)
+            seq_its.append(info_i["iterations"])
+            log(f"[batch]   member {i} ({specs[i].n[0]}^3) alone: {info_i['iterations']} "
+                f"iterations, {info_i['seconds']:.2f} s")
+        seq_s = time.perf_counter() - t0
+        share = len(sample) / len(specs)
+        si = {"tol": a.si_tol, "batched_seconds": si_s, "batched_iterations": sinfo["n_iterations"],
+              "member_iterations_min": int(min(its)), "member_iterations_max": int(max(its)),
+              "all_converged": all(it > 0 for it in its),
+              "sequential_sample_members": len(sample), "sequential_sample_seconds": seq_s,
+              "sequential_sample_iterations": seq_its,
+              "sequential_projected_seconds": seq_s / share if share > 0 else None}
+        log(f"[batch] batched SI to {a.si_tol:g}: {si_s:.2f} s, {sinfo['n_iterations']} "
+            f"iterations (members {min(its)}..{max(its)}); one at a time, {len(sample)} "
+            f"sampled members: {seq_s:.2f} s -> {si['sequential_projected_seconds']:.1f} s "
+            f"projected for the batch")
+
     gbs = alg_bytes / (graph_ms * 1e-3) / 1e9
     out = {
         "workload": "batch_instances",
@@ -118,7 +152,7 @@ def main(argv=None):
         "updates_per_s": updates / (graph_ms * 1e-3),
         "algorithmic_bytes": alg_bytes, "achieved_GBs": gbs, "roofline_frac": gbs / HBM_PEAK_GBS,
         "speedup_vs_sequential": (seq_ms / graph_ms) if seq_ms else None,
-        "tune_table": table, "bitwise_equal_spot_check": exact,
+        "tune_table": table, "bitwise_equal_spot_check": exact, "source_iteration": si,
         "device": ou.device_name(),
     }
     print(json.dumps(out))

@@ -3,6 +3,7 @@
 #include "batch.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -92,6 +93,12 @@ Batch::Batch(std::vector<RadiativeTransfer *> members, int waves, int dir_groups
     OU_HIP(hipEventCreateWithFlags(&io_done_, hipEventDisableTiming));
     OU_HIP(hipHostMalloc(reinterpret_cast<void **>(&h_io_), 2 * members_.size() * sizeof(double *)));
     OU_HIP(hipMalloc(reinterpret_cast<void **>(&d_io_), 2 * members_.size() * sizeof(double *)));
+    OU_HIP(hipHostMalloc(reinterpret_cast<void **>(&h_active_), members_.size() * sizeof(int)));
+    OU_HIP(hipMalloc(reinterpret_cast<void **>(&d_active_), members_.size() * sizeof(int)));
+    std::fill(h_active_, h_active_ + members_.size(), 1);
+    upload_active(stream_);
+    OU_HIP(hipStreamSynchronize(stream_));
+    bl_.active = d_active_;
     build_schedule();
     if (waves > 0 && dir_groups > 0) {
       set_shape(waves, dir_groups);
@@ -114,8 +121,18 @@ void Batch::release() {
   drop_graphs();
   for (void *p : {static_cast<void *>(d_chunks_), static_cast<void *>(d_chunk_view_),
                   static_cast<void *>(d_views_), static_cast<void *>(d_members_),
-                  static_cast<void *>(d_io_)})
+                  static_cast<void *>(d_io_), static_cast<void *>(d_si_work_),
+                  static_cast<void *>(d_si_partial_), static_cast<void *>(d_si_norms_),
+                  static_cast<void *>(d_active_)})
     if (p) (void)hipFree(p);
+  d_active_ = nullptr;
+  if (h_active_) (void)hipHostFree(h_active_);
+  h_active_ = nullptr;
+  d_si_work_ = nullptr;
+  d_si_partial_ = nullptr;
+  d_si_norms_ = nullptr;
+  if (h_si_norms_) (void)hipHostFree(h_si_norms_);
+  h_si_norms_ = nullptr;
   d_chunks_ = nullptr;
   d_chunk_view_ = nullptr;
   d_views_ = nullptr;
@@ -276,11 +293,18 @@ void Batch::upload_io(const double *const *xs, double *const *ys, hipStream_t s)
   io_pending_ = true;
 }
 
+void Batch::upload_active(hipStream_t s) {
+  OU_HIP(hipMemcpyAsync(d_active_, h_active_, members_.size() * sizeof(int),
+                        hipMemcpyHostToDevice, s));
+}
+
 void Batch::enqueue(int op, hipStream_t s) {
   const double *const *xs = d_io_;
   double *const *ys = const_cast<double *const *>(d_io_ + members_.size());
   if (op == 0)
     batch_apply(bl_, xs, ys, s);
+  else if (op == 3)
+    batch_si_step(bl_, xs, ys, d_si_partial_, kSiBlocks, d_si_norms_, s);
   else
     batch_sweep(bl_, ys, op == 2, s);
 }
@@ -321,6 +345,82 @@ void Batch::sweep(double *const *d_ys, bool with_source, hipStream_t s) {
   ensure_current();
   upload_io(nullptr, d_ys, s);
   run(with_source ? 2 : 1, s);
+}
+
+int Batch::solve_si(double *const *d_phi, double tol, int max_it, int *iterations,
+                    double *change, bool verbose) {
+  const size_t n = members_.size();
+  if (!d_phi || !iterations || !change) fail(OU_ERR_ARG, "null argument");
+  if (max_it < 1 || !(tol >= 0.0)) fail(OU_ERR_ARG, "bad tolerance or iteration limit");
+  ensure_current();
+  if (!d_si_work_) {
+    long total = 0;
+    for (const RadiativeTransfer *m : members_) total += m->plan().vec_size;
+    OU_HIP(hipMalloc(&d_si_work_, size_t(total) * 8));
+    OU_HIP(hipMalloc(&d_si_partial_, n * kSiBlocks * 2 * sizeof(double)));
+    OU_HIP(hipMalloc(&d_si_norms_, n * 2 * sizeof(double)));
+    OU_HIP(hipHostMalloc(reinterpret_cast<void **>(&h_si_norms_), n * 2 * sizeof(double)));
+  }
+  std::vector<double *> cur(n), nxt(n);
+  for (size_t u = 0, off = 0; u < n; off += members_[u]->plan().vec_size, ++u) {
+    if (!d_phi[u]) fail(OU_ERR_ARG, "null initial guess in the batch");
+    cur[u] = d_phi[u];
+    nxt[u] = d_si_work_ + off;
+    iterations[u] = -1;
+    change[u] = 0.0;
+  }
+  // A converged member keeps its last iterate: its flag is cleared, so the
+  // following steps skip its chunks and element-wise work, and its iterate
+  // is what single-plan source iteration returns at the same tolerance.
+  int it = 0;
+  try {
+    while (it < max_it) {
+      upload_io(cur.data(), nxt.data(), stream_);
+      run(3, stream_);
+      OU_HIP(hipMemcpyAsync(h_si_norms_, d_si_norms_, n * 2 * sizeof(double),
+                            hipMemcpyDeviceToHost, stream_));
+      OU_HIP(hipStreamSynchronize(stream_));
+      ++it;
+      size_t done = 0;
+      double worst = 0.0;
+      bool cleared = false;
+      for (size_t u = 0; u < n; ++u) {
+        if (iterations[u] >= 0) {
+          ++done;
+          continue;
+        }
+        const double d2 = h_si_norms_[2 * u], y2 = h_si_norms_[2 * u + 1];
+        change[u] = y2 > 0.0 ? std::sqrt(d2 / y2) : 0.0;
+        worst = std::max(worst, change[u]);
+        std::swap(cur[u], nxt[u]);
+        if (change[u] < tol) {
+          iterations[u] = it;
+          h_active_[u] = 0;
+          cleared = true;
+          ++done;
+        }
+      }
+      if (verbose && (it % 25 == 0 || done == n || it == max_it))
+        progress("batched SI iteration %d: largest open change %.3e, %zu of %zu members converged\n",
+                 it, worst, done, n);
+      if (done == n) break;
+      if (cleared) upload_active(stream_);  // ordered before the next step on this stream
+    }
+  } catch (...) {
+    (void)hipStreamSynchronize(stream_);
+    std::fill(h_active_, h_active_ + n, 1);
+    (void)hipMemcpy(d_active_, h_active_, n * sizeof(int), hipMemcpyHostToDevice);
+    throw;
+  }
+  std::fill(h_active_, h_active_ + n, 1);
+  upload_active(stream_);
+  // the last iterate of a member may sit in the work buffer
+  for (size_t u = 0; u < n; ++u)
+    if (cur[u] != d_phi[u])
+      OU_HIP(hipMemcpyAsync(d_phi[u], cur[u], size_t(members_[u]->plan().vec_size) * 8,
+                            hipMemcpyDeviceToDevice, stream_));
+  OU_HIP(hipStreamSynchronize(stream_));
+  return it;
 }
 
 std::string Batch::tune_key() const {

@@ -47,6 +47,17 @@ class Batch {
   // y_i += D L^-1 (M2D S_i [+ M q_i/ws + inflow]) for every member.
   void sweep(double *const *d_ys, bool with_source, hipStream_t s);
 
+  // Source iteration of every member from the initial guesses d_phi (device
+  // vectors, updated in place): phi_i <- D L^-1 (M2D S(phi_i) + M q_i/ws +
+  // inflow), one captured graph per iteration for the whole batch, until every
+  // member's change |phi_new - phi| / |phi_new| is below tol or max_it
+  // iterations.  iterations[i]: the iteration at which member i first met tol
+  // (-1 if it never did); change[i]: its last change.  Returns the number of
+  // iterations run.  Each member's iterates equal its single-plan source
+  // iteration's bit-for-bit.
+  int solve_si(double *const *d_phi, double tol, int max_it, int *iterations, double *change,
+               bool verbose);
+
   void set_shape(int waves, int dir_groups);
   void set_use_graph(bool on) { use_graph_ = on; }
   bool use_graph() const { return use_graph_; }
@@ -65,6 +76,7 @@ class Batch {
   void enqueue(int op, hipStream_t s);
   void drop_graphs();
   void release();
+  void upload_active(hipStream_t s);
 
   std::vector<RadiativeTransfer *> members_;
   BatchLaunch bl_;
@@ -79,7 +91,16 @@ class Batch {
   bool io_pending_ = false;
   std::vector<double *> seen_partial_;
   bool use_graph_ = true;
-  hipGraphExec_t exec_[3] = {nullptr, nullptr, nullptr};  // apply, sweep, sweep + source
+  // apply, sweep, sweep + source, source iteration step
+  hipGraphExec_t exec_[4] = {nullptr, nullptr, nullptr, nullptr};
+  // source iteration: second iterate per member, norm partials and sums
+  double *d_si_work_ = nullptr, *d_si_partial_ = nullptr, *d_si_norms_ = nullptr;
+  double *h_si_norms_ = nullptr;
+  // [members] flags read by every batched kernel; all ones outside solve_si,
+  // which clears a member's flag once it has converged
+  int *d_active_ = nullptr;
+  int *h_active_ = nullptr;  // pinned staging of the flags
+  static constexpr int kSiBlocks = 64;  // partial-sum blocks per member
 };
 
 }  // namespace ou

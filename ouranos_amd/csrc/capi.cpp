@@ -550,6 +550,16 @@ int ou_batch_sweep_device(ou_batch *batch, double *const *d_y, int with_source,
   });
 }
 
+int ou_batch_solve_si(ou_batch *batch, double *const *d_phi, double tol, int max_it,
+                      int *iterations, double *change, int *n_iterations, int verbose) {
+  return guarded([&] {
+    const int it =
+        batch_of(batch).solve_si(d_phi, tol, max_it, iterations, change, verbose != 0);
+    if (n_iterations) *n_iterations = it;
+    return int(OU_OK);
+  });
+}
+
 int ou_batch_tune(ou_batch *batch, int reps, ou_tune_entry *entries, int max_entries,
                   int *n_entries) {
   return guarded([&] {

@@ -144,6 +144,7 @@ struct BatchLaunch {
   std::vector<int> level_off;  // host
   const Chunk *chunks = nullptr;
   const int *chunk_view = nullptr;
+  const int *active = nullptr;           // [member]: 0 = skip the member's work
   const SweepView *views = nullptr;      // S0 = null (Apply, plain sweep)
   const SweepView *views_src = nullptr;  // S0 = member source (sweep with source)
   const MemberView *members = nullptr;
@@ -179,5 +180,10 @@ void batch_prepare(const BatchLaunch &b);
 void batch_apply(const BatchLaunch &b, const double *const *d_xs, double *const *d_ys,
                  hipStream_t s);
 void batch_sweep(const BatchLaunch &b, double *const *d_ys, bool with_source, hipStream_t s);
+// One source iteration of every member: S_i = S(x_i), y_i = D L^-1 (M2D S_i +
+// M q_i/ws + inflow), and d_norms[2i], d_norms[2i+1] = |y_i - x_i|^2, |y_i|^2
+// (d_partial: partial_blocks x 2 doubles per member).
+void batch_si_step(const BatchLaunch &b, const double *const *d_xs, double *const *d_ys,
+                   double *d_partial, int partial_blocks, double *d_norms, hipStream_t s);
 
 }  // namespace ou

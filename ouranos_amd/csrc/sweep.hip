@@ -274,7 +274,8 @@ __global__ __launch_bounds__(64) void k_ainv_to_mfma64(const double *__restrict_
     }
 }
 
-// y = x - sum_o partial_o (Apply) or y += sum_o partial_o (sweep), summed
+// y = x - sum_o partial_o (Apply, mode 1), y += sum_o partial_o (sweep, mode
+// 0) or y = 0 + sum_o partial_o (a sweep into a fresh vector, mode 2), summed
 // over the n_parts = octants x direction groups partial moment vectors.
 __device__ __forceinline__ void epilogue_one(long gid, int n_parts, long stride,
                                              const double *__restrict__ partial,
@@ -282,7 +283,7 @@ __device__ __forceinline__ void epilogue_one(long gid, int n_parts, long stride,
                                              double *__restrict__ y, int apply) {
   double s = 0.0;
   for (int o = 0; o < n_parts; ++o) s += partial[o * stride + gid];
-  y[gid] = apply ? x[gid] - s : y[gid] + s;
+  y[gid] = apply == 1 ? x[gid] - s : (apply == 2 ? 0.0 + s : y[gid] + s);
 }
 
 __global__ __launch_bounds__(256) void k_epilogue(long n, int n_parts, long stride,
@@ -301,10 +302,11 @@ __global__ __launch_bounds__(256) void k_epilogue(long n, int n_parts, long stri
 // ------------------------------------------------------------------------
 template <typename T, int N>
 __global__ __launch_bounds__(256) void k_batch_scatter(const MemberView *__restrict__ mv,
-                                                       int n_mem,
+                                                       int n_mem, const int *__restrict__ active,
                                                        const double *const *__restrict__ xs) {
   const long stride = long(gridDim.x) * blockDim.x;
   for (int u = 0; u < n_mem; ++u) {
+    if (!active[u]) continue;
     const MemberView &m = mv[u];
     const long n = m.n_cells * m.n_mom;
     for (long gid = long(blockIdx.x) * blockDim.x + threadIdx.x; gid < n; gid += stride)
@@ -315,9 +317,10 @@ __global__ __launch_bounds__(256) void k_batch_scatter(const MemberView *__restr
 
 template <typename T, int N>
 __global__ __launch_bounds__(256) void k_batch_source0(const MemberView *__restrict__ mv,
-                                                       int n_mem) {
+                                                       int n_mem, const int *__restrict__ active) {
   const long stride = long(gridDim.x) * blockDim.x;
   for (int u = 0; u < n_mem; ++u) {
+    if (!active[u]) continue;
     const MemberView &m = mv[u];
     const long n = m.n_cells * N;
     for (long gid = long(blockIdx.x) * blockDim.x + threadIdx.x; gid < n; gid += stride)
@@ -328,9 +331,11 @@ __global__ __launch_bounds__(256) void k_batch_source0(const MemberView *__restr
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_batch_trace_init(const MemberView *__restrict__ mv,
-                                                          int n_mem, int dim, int with_source) {
+                                                          int n_mem, const int *__restrict__ active,
+                                                          int dim, int with_source) {
   const long stride = long(gridDim.x) * blockDim.x;
   for (int u = 0; u < n_mem; ++u) {
+    if (!active[u]) continue;
     const MemberView &m = mv[u];
     for (int a = 0; a < dim; ++a) {
       T *tr = static_cast<T *>(m.trace[a]);
@@ -342,18 +347,74 @@ __global__ __launch_bounds__(256) void k_batch_trace_init(const MemberView *__re
 }
 
 __global__ __launch_bounds__(256) void k_batch_epilogue(const MemberView *__restrict__ mv,
-                                                        int n_mem,
+                                                        int n_mem, const int *__restrict__ active,
                                                         const double *const *__restrict__ xs,
                                                         double *const *__restrict__ ys,
                                                         int apply) {
   const long stride = long(gridDim.x) * blockDim.x;
   for (int u = 0; u < n_mem; ++u) {
+    if (!active[u]) continue;
     const MemberView &m = mv[u];
-    const double *x = apply ? xs[u] : nullptr;
+    const double *x = apply == 1 ? xs[u] : nullptr;
     double *y = ys[u];
     for (long gid = long(blockIdx.x) * blockDim.x + threadIdx.x; gid < m.vec_size;
          gid += stride)
       epilogue_one(gid, m.n_parts, m.vec_size, m.partial, x, y, apply);
+  }
+}
+
+// Per member and block: sum (y - x)^2 and sum y^2 over a grid-strided share
+// of the member's vector (blockIdx.y = member), for the source-iteration
+// change ||phi_new - phi|| / ||phi_new||.
+__global__ __launch_bounds__(256) void k_batch_change_norms(const MemberView *__restrict__ mv,
+                                                            const int *__restrict__ active,
+                                                            const double *const *__restrict__ xs,
+                                                            const double *const *__restrict__ ys,
+                                                            double *__restrict__ partial) {
+  __shared__ double red[2][4];
+  const int u = blockIdx.y;
+  if (!active[u]) return;
+  const long n = mv[u].vec_size;
+  const double *x = xs[u], *y = ys[u];
+  double d2 = 0.0, y2 = 0.0;
+  for (long k = long(blockIdx.x) * blockDim.x + threadIdx.x; k < n;
+       k += long(gridDim.x) * blockDim.x) {
+    const double yk = y[k], dk = yk - x[k];
+    d2 += dk * dk;
+    y2 += yk * yk;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    d2 += __shfl_xor(d2, o);
+    y2 += __shfl_xor(y2, o);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = d2;
+    red[1][threadIdx.x >> 6] = y2;
+  }
+  __syncthreads();
+  if (threadIdx.x < 2) {
+    double t = 0.0;
+    for (int q = 0; q < 4; ++q) t += red[threadIdx.x][q];
+    partial[(long(u) * gridDim.x + blockIdx.x) * 2 + threadIdx.x] = t;
+  }
+}
+
+// out[u][0..1] = sums of the nb partials of member u (blockIdx.x = member).
+__global__ __launch_bounds__(64) void k_batch_sum_pairs(const double *__restrict__ partial,
+                                                        int nb, double *__restrict__ out) {
+  const int u = blockIdx.x;
+  double a = 0.0, b = 0.0;
+  for (int k = threadIdx.x; k < nb; k += 64) {
+    a += partial[(long(u) * nb + k) * 2];
+    b += partial[(long(u) * nb + k) * 2 + 1];
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    a += __shfl_xor(a, o);
+    b += __shfl_xor(b, o);
+  }
+  if (threadIdx.x == 0) {
+    out[2 * u] = a;
+    out[2 * u + 1] = b;
   }
 }
 
@@ -412,7 +473,7 @@ void launch_batch_levels_impl(const BatchLaunch &bl, const SweepView *views, hip
     const int b = bl.level_off[l], e = bl.level_off[l + 1];
     if (e > b)
       hipLaunchKernelGGL(kern, dim3(e - b, groups), dim3(64 * W), C::lds_bytes, s, views,
-                         bl.chunk_view, bl.chunks, b);
+                         bl.chunk_view, bl.chunks, b, bl.active);
   }
   OU_HIP(hipGetLastError());
 }
@@ -678,7 +739,7 @@ void batch_scatter_impl(const BatchLaunch &b, const double *const *d_xs, hipStre
 #define OU_BSCAT(NN)                                                                   \
   case NN:                                                                             \
     hipLaunchKernelGGL((k_batch_scatter<T, NN>), dim3(g), dim3(256), 0, s, b.members, \
-                       b.n_members, d_xs);                                             \
+                       b.n_members, b.active, d_xs);                                   \
     break;
   switch (b.N) {
     OU_BSCAT(4) OU_BSCAT(9) OU_BSCAT(16) OU_BSCAT(8) OU_BSCAT(27)
@@ -693,7 +754,7 @@ void batch_source0_impl(const BatchLaunch &b, hipStream_t s) {
 #define OU_BSRC(NN)                                                                    \
   case NN:                                                                             \
     hipLaunchKernelGGL((k_batch_source0<T, NN>), dim3(g), dim3(256), 0, s, b.members, \
-                       b.n_members);                                                   \
+                       b.n_members, b.active);                                         \
     break;
   switch (b.N) {
     OU_BSRC(4) OU_BSRC(9) OU_BSRC(16) OU_BSRC(8) OU_BSRC(27)
@@ -706,16 +767,16 @@ void batch_trace_init(const BatchLaunch &b, bool with_source, hipStream_t s) {
   const unsigned g = batch_grid(b);
   if (b.precision == OU_FP32)
     hipLaunchKernelGGL(k_batch_trace_init<float>, dim3(g), dim3(256), 0, s, b.members,
-                       b.n_members, b.dim, int(with_source));
+                       b.n_members, b.active, b.dim, int(with_source));
   else
     hipLaunchKernelGGL(k_batch_trace_init<double>, dim3(g), dim3(256), 0, s, b.members,
-                       b.n_members, b.dim, int(with_source));
+                       b.n_members, b.active, b.dim, int(with_source));
 }
 
 void batch_epilogue(const BatchLaunch &b, const double *const *d_xs, double *const *d_ys,
-                    bool apply, hipStream_t s) {
+                    int mode, hipStream_t s) {
   hipLaunchKernelGGL(k_batch_epilogue, dim3(batch_grid(b)), dim3(256), 0, s, b.members,
-                     b.n_members, d_xs, d_ys, int(apply));
+                     b.n_members, b.active, d_xs, d_ys, mode);
 }
 
 }  // namespace
@@ -731,7 +792,7 @@ void batch_apply(const BatchLaunch &b, const double *const *d_xs, double *const 
     batch_scatter_impl<double>(b, d_xs, s);
   batch_trace_init(b, false, s);
   k->batch(b, b.views, s, true);
-  batch_epilogue(b, d_xs, d_ys, true, s);
+  batch_epilogue(b, d_xs, d_ys, 1, s);
   OU_HIP(hipGetLastError());
 }
 
@@ -745,7 +806,28 @@ void batch_sweep(const BatchLaunch &b, double *const *d_ys, bool with_source, hi
   }
   batch_trace_init(b, with_source, s);
   k->batch(b, with_source ? b.views_src : b.views, s, true);
-  batch_epilogue(b, nullptr, d_ys, false, s);
+  batch_epilogue(b, nullptr, d_ys, 0, s);
+  OU_HIP(hipGetLastError());
+}
+
+void batch_si_step(const BatchLaunch &b, const double *const *d_xs, double *const *d_ys,
+                   double *d_partial, int partial_blocks, double *d_norms, hipStream_t s) {
+  const LevelKernels *k = batch_kernels(b);
+  if (b.precision == OU_FP32) {
+    batch_scatter_impl<float>(b, d_xs, s);
+    batch_source0_impl<float>(b, s);
+  } else {
+    batch_scatter_impl<double>(b, d_xs, s);
+    batch_source0_impl<double>(b, s);
+  }
+  batch_trace_init(b, true, s);
+  k->batch(b, b.views_src, s, true);
+  batch_epilogue(b, nullptr, d_ys, 2, s);
+  hipLaunchKernelGGL(k_batch_change_norms, dim3(unsigned(partial_blocks), unsigned(b.n_members)),
+                     dim3(256), 0, s, b.members, b.active, d_xs,
+                     static_cast<const double *const *>(d_ys), d_partial);
+  hipLaunchKernelGGL(k_batch_sum_pairs, dim3(unsigned(b.n_members)), dim3(64), 0, s, d_partial,
+                     partial_blocks, d_norms);
   OU_HIP(hipGetLastError());
 }
 

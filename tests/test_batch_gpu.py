@@ -194,6 +194,51 @@ def test_batch_autotuner_picks_fastest_and_stays_exact():
     batch.close()
 
 
+@pytest.mark.parametrize("precision", [ou.FP64, ou.FP32])
+def test_batched_source_iteration_equals_single(precision):
+    """Each member's batched source-iteration iterates equal its own ou_solve
+    SI iterates bit-for-bit (fixed iteration count), and the converged run
+    meets the tolerance for every member."""
+    specs = _uneven_specs()
+    ops = [s.operator(precision) for s in specs]
+    batch = ou.Batch(ops, waves=2, dir_groups=1)
+    k = 6
+    phis, info = batch.solve_si(tol=0.0, max_it=k)
+    print(f"[gpu] batched SI, {k} iterations: {info['n_iterations']}", flush=True)
+    assert info["n_iterations"] == k
+    for i, (op, phi) in enumerate(zip(ops, phis)):
+        ref, rinfo = op.solve(solver=ou.SI, tol=1e-300, max_it=k, raise_on_fail=False)
+        assert rinfo["iterations"] == k
+        assert np.array_equal(phi, ref), (i, rel_err(phi, ref))
+    phis, info = batch.solve_si(tol=1e-9, max_it=2000)
+    print(f"[gpu] batched SI to 1e-9: {info['n_iterations']} iterations, "
+          f"per member {info['iterations']}", flush=True)
+    assert all(0 < it <= info["n_iterations"] for it in info["iterations"])
+    assert max(info["iterations"]) == info["n_iterations"]
+    assert len(set(info["iterations"])) > 1  # members converge at different steps
+    # a converged member stops: its result is its own SI after exactly that
+    # many iterations, and single-plan SI at the same tolerance stops there too
+    # (the two compute the change norm in different summation orders, so a
+    # borderline step may differ by one)
+    for i, (op, phi) in enumerate(zip(ops, phis)):
+        its = info["iterations"][i]
+        ref, _ = op.solve(solver=ou.SI, tol=1e-300, max_it=its, raise_on_fail=False)
+        assert np.array_equal(phi, ref), (i, its, rel_err(phi, ref))
+        _, rinfo = op.solve(solver=ou.SI, tol=1e-9, max_it=2000)
+        assert abs(rinfo["iterations"] - its) <= 1, (i, its, rinfo["iterations"])
+    # GMRES on the same operator (FP32 operators reach about 1e-7)
+    gtol, tol = (1e-11, 1e-6) if precision == ou.FP64 else (1e-6, 1e-5)
+    for op, phi in zip(ops, phis):
+        ref, _ = op.solve(solver=ou.GMRES, tol=gtol, max_it=2000)
+        assert rel_err(phi, ref) < tol
+    # every member is active again afterwards: a batched Apply covers all
+    xs = _vectors(specs, 11)
+    ys = batch.apply(xs)
+    for i, (op, x, y) in enumerate(zip(ops, xs, ys)):
+        assert np.array_equal(y, op.Apply(x)), i
+    batch.close()
+
+
 def test_batch_errors():
     specs = _specs(n_instances=1, n_rhs=2, sizes=(4,))
     ops = [s.operator(ou.FP64) for s in specs]
