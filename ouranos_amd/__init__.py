@@ -144,6 +144,8 @@ def lib():
                                        ctypes.c_double, _dp, ctypes.POINTER(SolveInfo)]
         L.ou_set_sweep_shape.argtypes = [_vp, ctypes.c_int, ctypes.c_int]
         L.ou_sweep_shape.argtypes = [_vp, _ip, _ip]
+        L.ou_set_sweep_streams.argtypes = [_vp, ctypes.c_int]
+        L.ou_sweep_streams.argtypes = [_vp, _ip]
         L.ou_batch_create.argtypes = [ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int,
                                       ctypes.c_int, ctypes.POINTER(_vp)]
         L.ou_batch_destroy.argtypes = [_vp]
@@ -488,6 +490,17 @@ class RadiativeTransfer:
     @sweep_shape.setter
     def sweep_shape(self, shape):
         _check(lib().ou_set_sweep_shape(self._plan, int(shape[0]), int(shape[1])))
+
+    @property
+    def sweep_streams(self):
+        """1: one launch per wavefront level; 2: one per octant half, on two streams."""
+        n = ctypes.c_int()
+        _check(lib().ou_sweep_streams(self._plan, ctypes.byref(n)))
+        return n.value
+
+    @sweep_streams.setter
+    def sweep_streams(self, n):
+        _check(lib().ou_set_sweep_streams(self._plan, int(n)))
 
     # --- device API --------------------------------------------------------------
     def apply_device(self, d_x, d_y):

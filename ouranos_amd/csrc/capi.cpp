@@ -453,8 +453,11 @@ int ou_time_apply(ou_plan *plan, const double *d_x, double *d_y, int warmup, int
       ms_sweep += ms;
     }
     int launches = 0;
-    for (int l = 0; l < pl.n_levels; ++l)
-      launches += pl.level_chunk_off[l + 1] > pl.level_chunk_off[l];
+    for (int l = 0; l < pl.n_levels; ++l) {
+      const int b = pl.level_chunk_off[l], e = pl.level_chunk_off[l + 1];
+      const int m = pl.sweep_streams == 2 ? pl.level_split[l] : e;
+      launches += (m > b) + (e > m);
+    }
     timing->ms_total = ms_total;
     timing->ms_sweep_kernel = ms_sweep;
     timing->ms_other = ms_total - ms_sweep;
@@ -480,6 +483,24 @@ int ou_sweep_shape(const ou_plan *plan, int *waves, int *dir_groups) {
     const ou::Plan &pl = plan_of(const_cast<ou_plan *>(plan));
     if (waves) *waves = pl.waves;
     if (dir_groups) *dir_groups = pl.dir_groups;
+    return int(OU_OK);
+  });
+}
+
+int ou_set_sweep_streams(ou_plan *plan, int n_streams) {
+  return guarded([&] {
+    if (n_streams != 1 && n_streams != 2) ou::fail(OU_ERR_ARG, "sweep streams must be 1 or 2");
+    ou::Plan &pl = plan_of(plan);
+    OU_HIP(hipStreamSynchronize(pl.stream));
+    pl.sweep_streams = n_streams;
+    return int(OU_OK);
+  });
+}
+
+int ou_sweep_streams(const ou_plan *plan, int *n_streams) {
+  return guarded([&] {
+    if (!n_streams) ou::fail(OU_ERR_ARG, "null output pointer");
+    *n_streams = plan_of(const_cast<ou_plan *>(plan)).sweep_streams;
     return int(OU_OK);
   });
 }

@@ -74,6 +74,9 @@ Plan::~Plan() {
     if (b) (void)hipFree(b);
   for (hipEvent_t e : ev)
     if (e) (void)hipEventDestroy(e);
+  for (hipEvent_t e : {ev_fork, ev_join})
+    if (e) (void)hipEventDestroy(e);
+  if (stream2) (void)hipStreamDestroy(stream2);
   if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -281,30 +284,44 @@ Plan *create_plan(const ou_problem_desc *desc) {
         }
       }
     }
-    // Order each level's chunks so that all chunks of one material (every
-    // octant uses the same inverse classes) land on the same XCD: workgroup i
-    // of a launch runs on XCD i mod 8, each XCD with its own L2.  Chunks are
-    // bucketed by material mod 8 and dealt out so position p draws from
-    // bucket p mod 8 (from the fullest bucket once that one is empty).
+    // Each level lists the chunks of the first octant half (octants
+    // 0 .. n_oct/2-1), then those of the second: the two halves are
+    // independent and may be launched on two streams, so that one half's
+    // level fills the CUs the other's leaves idle while its last workgroups
+    // drain.  Within a half, chunks are ordered so that all chunks of one
+    // material (every octant uses the same inverse classes) land on the same
+    // XCD: workgroup i of a launch runs on XCD i mod 8, each XCD with its own
+    // L2.  Chunks are bucketed by material mod 8 and dealt out so position p
+    // draws from bucket p mod 8 (from the fullest bucket once that one is
+    // empty).
     constexpr int kXcds = 8;
     std::vector<Chunk> chunks;
     p.level_chunk_off.assign(p.n_levels + 1, 0);
+    p.level_split.assign(p.n_levels, 0);
     for (int l = 0; l < p.n_levels; ++l) {
       p.level_chunk_off[l] = int(chunks.size());
-      std::vector<std::vector<Chunk>> xb(kXcds);
-      for (const Chunk &c : per_level[l]) xb[c.material % kXcds].push_back(c);
-      std::vector<size_t> next(kXcds, 0);
-      for (size_t pos = 0; pos < per_level[l].size(); ++pos) {
-        int b = int(pos % kXcds);
-        if (next[b] == xb[b].size()) {
-          size_t best = 0;
-          for (int q = 0; q < kXcds; ++q)
-            if (xb[q].size() - next[q] > best) {
-              best = xb[q].size() - next[q];
-              b = q;
-            }
+      for (int half = 0; half < 2; ++half) {
+        if (half == 1) p.level_split[l] = int(chunks.size());
+        std::vector<std::vector<Chunk>> xb(kXcds);
+        size_t n_half = 0;
+        for (const Chunk &c : per_level[l])
+          if ((c.octant >= p.n_oct / 2) == (half == 1)) {
+            xb[c.material % kXcds].push_back(c);
+            ++n_half;
+          }
+        std::vector<size_t> next(kXcds, 0);
+        for (size_t pos = 0; pos < n_half; ++pos) {
+          int b = int(pos % kXcds);
+          if (next[b] == xb[b].size()) {
+            size_t best = 0;
+            for (int q = 0; q < kXcds; ++q)
+              if (xb[q].size() - next[q] > best) {
+                best = xb[q].size() - next[q];
+                b = q;
+              }
+          }
+          chunks.push_back(xb[b][next[b]++]);
         }
-        chunks.push_back(xb[b][next[b]++]);
       }
     }
     p.level_chunk_off[p.n_levels] = int(chunks.size());
@@ -356,7 +373,10 @@ Plan *create_plan(const ou_problem_desc *desc) {
 
     // ---- device buffers -----------------------------------------------------
     OU_HIP(hipStreamCreateWithFlags(&p.stream, hipStreamNonBlocking));
+    OU_HIP(hipStreamCreateWithFlags(&p.stream2, hipStreamNonBlocking));
     for (auto &e : p.ev) OU_HIP(hipEventCreate(&e));
+    OU_HIP(hipEventCreateWithFlags(&p.ev_fork, hipEventDisableTiming));
+    OU_HIP(hipEventCreateWithFlags(&p.ev_join, hipEventDisableTiming));
     const size_t rs = p.real_size();
     p.d_ainv = dev_alloc(size_t(p.n_mat) * p.n_cls * p.N * p.N * rs);
     if (p.use_mfma)

@@ -185,6 +185,31 @@ def test_multi_column_apply_equals_columns():
         op.Apply(np.zeros((2, spec.size - 1)))
 
 
+@pytest.mark.parametrize("precision", [ou.FP64, ou.FP32])
+@pytest.mark.parametrize("name", ["hetero_2d_q2_L3", "batch_3d_q1", "headline_3d_q2_L1"])
+def test_octant_halves_on_two_streams_equal_one(name, precision):
+    """Each level as two launches (octant halves, two streams) or one: the
+    same chunks run the same arithmetic, so Apply is identical bit-for-bit;
+    the source sweep too."""
+    spec = CASES[name]()
+    op = spec.operator(precision)
+    assert op.sweep_streams == 2  # the default
+    xs = [P.seeded_vector(spec.size, 60 + r) for r in range(3)]
+    two = [op.Apply(x) for x in xs]
+    op.compute_scattering_source(xs[0])
+    two_src = op.sweep(None, with_source=True)
+    op.sweep_streams = 1
+    assert op.sweep_streams == 1
+    for x, y2 in zip(xs, two):
+        assert np.array_equal(op.Apply(x), y2)
+    op.compute_scattering_source(xs[0])
+    assert np.array_equal(op.sweep(None, with_source=True), two_src)
+    op.sweep_streams = 2
+    assert np.array_equal(op.Apply(xs[1]), two[1])
+    with pytest.raises(ou.OuError):
+        op.sweep_streams = 3
+
+
 def test_apply_properties_at_full_2d_size():
     """256x256 Q2, 256 directions, L=3: linearity and sigma_s = 0 => identity."""
     spec = P.hetero_2d()
