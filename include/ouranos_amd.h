@@ -306,11 +306,12 @@ int ou_solve_refined(ou_plan *plan64, ou_plan *plan32, ou_unet *net,
 int ou_set_sweep_shape(ou_plan *plan, int waves, int dir_groups);
 int ou_sweep_shape(const ou_plan *plan, int *waves, int *dir_groups);
 
-/* Streams of one sweep: 2 (the default) launches each level as two kernels,
- * one per octant half, on the plan's stream and a second stream forked from
- * and joined back into it, so that one half's level fills the CUs the other
- * half's leaves idle while its last workgroups drain; 1 launches each level
- * as one kernel.  Results are bit-for-bit the same.  No reference
+/* Streams of one sweep: n_streams = 2 (the default) launches each level as
+ * two kernels, one per octant half, on the plan's stream and a side stream
+ * forked from and joined back into it, so that one half's level fills the
+ * CUs the other half's leaves idle while its last workgroups drain; 4 does
+ * the same with octant quarters on four streams; 1 launches each level as
+ * one kernel.  Results are bit-for-bit the same.  No reference
  * counterpart (the reference's scheduler runs tasks as their inputs arrive,
  * RadiativeTransfer.cc:605-778). */
 int ou_set_sweep_streams(ou_plan *plan, int n_streams);

@@ -493,7 +493,8 @@ class RadiativeTransfer:
 
     @property
     def sweep_streams(self):
-        """1: one launch per wavefront level; 2: one per octant half, on two streams."""
+        """Launches per wavefront level, each on its own stream: 1, 2 (octant
+        halves) or 4 (octant quarters)."""
         n = ctypes.c_int()
         _check(lib().ou_sweep_streams(self._plan, ctypes.byref(n)))
         return n.value

@@ -453,11 +453,10 @@ int ou_time_apply(ou_plan *plan, const double *d_x, double *d_y, int warmup, int
       ms_sweep += ms;
     }
     int launches = 0;
-    for (int l = 0; l < pl.n_levels; ++l) {
-      const int b = pl.level_chunk_off[l], e = pl.level_chunk_off[l + 1];
-      const int m = pl.sweep_streams == 2 ? pl.level_split[l] : e;
-      launches += (m > b) + (e > m);
-    }
+    const int ns = pl.sweep_streams;
+    for (int l = 0; l < pl.n_levels; ++l)
+      for (int k = 0; k < ns; ++k)
+        launches += pl.quarter_begin(l, 4 * (k + 1) / ns) > pl.quarter_begin(l, 4 * k / ns);
     timing->ms_total = ms_total;
     timing->ms_sweep_kernel = ms_sweep;
     timing->ms_other = ms_total - ms_sweep;
@@ -489,7 +488,8 @@ int ou_sweep_shape(const ou_plan *plan, int *waves, int *dir_groups) {
 
 int ou_set_sweep_streams(ou_plan *plan, int n_streams) {
   return guarded([&] {
-    if (n_streams != 1 && n_streams != 2) ou::fail(OU_ERR_ARG, "sweep streams must be 1 or 2");
+    if (n_streams != 1 && n_streams != 2 && n_streams != 4)
+      ou::fail(OU_ERR_ARG, "sweep streams must be 1, 2 or 4");
     ou::Plan &pl = plan_of(plan);
     OU_HIP(hipStreamSynchronize(pl.stream));
     pl.sweep_streams = n_streams;

@@ -74,9 +74,10 @@ Plan::~Plan() {
     if (b) (void)hipFree(b);
   for (hipEvent_t e : ev)
     if (e) (void)hipEventDestroy(e);
-  for (hipEvent_t e : {ev_fork, ev_join})
+  for (hipEvent_t e : {ev_fork, ev_join[0], ev_join[1], ev_join[2]})
     if (e) (void)hipEventDestroy(e);
-  if (stream2) (void)hipStreamDestroy(stream2);
+  for (hipStream_t s : side)
+    if (s) (void)hipStreamDestroy(s);
   if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -284,11 +285,11 @@ Plan *create_plan(const ou_problem_desc *desc) {
         }
       }
     }
-    // Each level lists the chunks of the first octant half (octants
-    // 0 .. n_oct/2-1), then those of the second: the two halves are
-    // independent and may be launched on two streams, so that one half's
-    // level fills the CUs the other's leaves idle while its last workgroups
-    // drain.  Within a half, chunks are ordered so that all chunks of one
+    // Each level lists its chunks by octant quarter (octants q*n_oct/4 ..):
+    // octants are independent, so the parts of a level may be launched on
+    // separate streams, and one part's level fills the CUs another's leaves
+    // idle while its last workgroups drain.  Within a quarter, chunks are
+    // ordered so that all chunks of one
     // material (every octant uses the same inverse classes) land on the same
     // XCD: workgroup i of a launch runs on XCD i mod 8, each XCD with its own
     // L2.  Chunks are bucketed by material mod 8 and dealt out so position p
@@ -297,15 +298,15 @@ Plan *create_plan(const ou_problem_desc *desc) {
     constexpr int kXcds = 8;
     std::vector<Chunk> chunks;
     p.level_chunk_off.assign(p.n_levels + 1, 0);
-    p.level_split.assign(p.n_levels, 0);
+    p.level_quarter.assign(4 * size_t(p.n_levels), 0);
     for (int l = 0; l < p.n_levels; ++l) {
       p.level_chunk_off[l] = int(chunks.size());
-      for (int half = 0; half < 2; ++half) {
-        if (half == 1) p.level_split[l] = int(chunks.size());
+      for (int q = 0; q < 4; ++q) {
+        p.level_quarter[4 * l + q] = int(chunks.size());
         std::vector<std::vector<Chunk>> xb(kXcds);
         size_t n_half = 0;
         for (const Chunk &c : per_level[l])
-          if ((c.octant >= p.n_oct / 2) == (half == 1)) {
+          if (c.octant * 4 / p.n_oct == q) {
             xb[c.material % kXcds].push_back(c);
             ++n_half;
           }
@@ -373,10 +374,10 @@ Plan *create_plan(const ou_problem_desc *desc) {
 
     // ---- device buffers -----------------------------------------------------
     OU_HIP(hipStreamCreateWithFlags(&p.stream, hipStreamNonBlocking));
-    OU_HIP(hipStreamCreateWithFlags(&p.stream2, hipStreamNonBlocking));
+    for (auto &s : p.side) OU_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     for (auto &e : p.ev) OU_HIP(hipEventCreate(&e));
     OU_HIP(hipEventCreateWithFlags(&p.ev_fork, hipEventDisableTiming));
-    OU_HIP(hipEventCreateWithFlags(&p.ev_join, hipEventDisableTiming));
+    for (auto &e : p.ev_join) OU_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     const size_t rs = p.real_size();
     p.d_ainv = dev_alloc(size_t(p.n_mat) * p.n_cls * p.N * p.N * rs);
     if (p.use_mfma)

@@ -87,18 +87,23 @@ struct Plan {
   std::vector<double> cls_omega;     // [cls][3] |Omega| of the class
   int *d_slot_cls = nullptr;
   std::vector<int> level_chunk_off;  // [n_levels+1]
-  // [n_levels]: first chunk of level l whose octant is in the second half
-  // (octants n_oct/2 ..); each level lists the first half's chunks first
-  std::vector<int> level_split;
-  int sweep_streams = 2;  // 1: one launch per level; 2: one per octant half
+  // Each level lists its chunks by octant quarter (quarter q holds octants
+  // q*n_oct/4 ..); level_quarter[4*l + q] is the first chunk of quarter q
+  // (level_chunk_off[l] for q = 0, level_chunk_off[l+1] for q = 4 is implied).
+  std::vector<int> level_quarter;
+  int sweep_streams = 2;  // launches per level: 1, 2 (octant halves) or 4 (quarters)
+  int quarter_begin(int l, int q) const {
+    return q == 0 ? level_chunk_off[l] : q == 4 ? level_chunk_off[l + 1] : level_quarter[4 * l + q];
+  }
   long n_chunks = 0;
 
   // device buffers
   hipStream_t stream = nullptr;
-  // the second octant half's launches, forked from and joined back into the
+  // the other octant parts' launches, forked from and joined back into the
   // sweep's stream by events
-  hipStream_t stream2 = nullptr;
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  hipStream_t side[3] = {nullptr, nullptr, nullptr};
+  hipEvent_t ev_fork = nullptr;
+  hipEvent_t ev_join[3] = {nullptr, nullptr, nullptr};
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   void *d_ainv = nullptr, *d_S = nullptr, *d_S0 = nullptr;
   float *d_ainv_mfma = nullptr;

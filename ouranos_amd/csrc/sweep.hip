@@ -448,26 +448,26 @@ void launch_levels_impl(const Plan &pl, const SweepView &v, hipStream_t s,
     attr_set = true;
   }
   if (before) OU_HIP(hipEventRecord(before, s));
-  // Two octant halves on two streams: the second half's launches wait for
+  // Octant parts on separate streams: the side streams' launches wait for
   // what s holds before the sweep, and s waits for them before what follows.
-  const bool split = pl.sweep_streams == 2;
-  hipStream_t s2 = pl.stream2;
-  if (split) {
+  const int ns = pl.sweep_streams;
+  if (ns > 1) {
     OU_HIP(hipEventRecord(pl.ev_fork, s));
-    OU_HIP(hipStreamWaitEvent(s2, pl.ev_fork, 0));
+    for (int k = 0; k < ns - 1; ++k) OU_HIP(hipStreamWaitEvent(pl.side[k], pl.ev_fork, 0));
   }
   const unsigned groups = unsigned(pl.dir_groups);
-  for (int l = 0; l < pl.n_levels; ++l) {
-    const int b = pl.level_chunk_off[l], e = pl.level_chunk_off[l + 1];
-    const int m = split ? pl.level_split[l] : e;
-    if (m > b)
-      hipLaunchKernelGGL(kern, dim3(m - b, groups), dim3(64 * W), C::lds_bytes, s, v, b);
-    if (e > m)
-      hipLaunchKernelGGL(kern, dim3(e - m, groups), dim3(64 * W), C::lds_bytes, s2, v, m);
-  }
-  if (split) {
-    OU_HIP(hipEventRecord(pl.ev_join, s2));
-    OU_HIP(hipStreamWaitEvent(s, pl.ev_join, 0));
+  for (int l = 0; l < pl.n_levels; ++l)
+    for (int k = 0; k < ns; ++k) {
+      const int b = pl.quarter_begin(l, 4 * k / ns), e = pl.quarter_begin(l, 4 * (k + 1) / ns);
+      if (e > b)
+        hipLaunchKernelGGL(kern, dim3(e - b, groups), dim3(64 * W), C::lds_bytes,
+                           k == 0 ? s : pl.side[k - 1], v, b);
+    }
+  if (ns > 1) {
+    for (int k = 0; k < ns - 1; ++k) {
+      OU_HIP(hipEventRecord(pl.ev_join[k], pl.side[k]));
+      OU_HIP(hipStreamWaitEvent(s, pl.ev_join[k], 0));
+    }
   }
   if (after) OU_HIP(hipEventRecord(after, s));
   OU_HIP(hipGetLastError());

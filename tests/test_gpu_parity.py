@@ -187,10 +187,10 @@ def test_multi_column_apply_equals_columns():
 
 @pytest.mark.parametrize("precision", [ou.FP64, ou.FP32])
 @pytest.mark.parametrize("name", ["hetero_2d_q2_L3", "batch_3d_q1", "headline_3d_q2_L1"])
-def test_octant_halves_on_two_streams_equal_one(name, precision):
-    """Each level as two launches (octant halves, two streams) or one: the
-    same chunks run the same arithmetic, so Apply is identical bit-for-bit;
-    the source sweep too."""
+def test_octant_parts_on_several_streams_equal_one(name, precision):
+    """Each level as one launch, two (octant halves, two streams) or four
+    (quarters): the same chunks run the same arithmetic, so Apply is
+    identical bit-for-bit; the source sweep too."""
     spec = CASES[name]()
     op = spec.operator(precision)
     assert op.sweep_streams == 2  # the default
@@ -204,8 +204,10 @@ def test_octant_halves_on_two_streams_equal_one(name, precision):
         assert np.array_equal(op.Apply(x), y2)
     op.compute_scattering_source(xs[0])
     assert np.array_equal(op.sweep(None, with_source=True), two_src)
-    op.sweep_streams = 2
+    op.sweep_streams = 4
     assert np.array_equal(op.Apply(xs[1]), two[1])
+    op.sweep_streams = 2
+    assert np.array_equal(op.Apply(xs[2]), two[2])
     with pytest.raises(ou.OuError):
         op.sweep_streams = 3
 
