@@ -342,6 +342,12 @@ int ou_batch_info(const ou_batch *batch, int *waves, int *dir_groups, int *n_lev
 int ou_batch_set_shape(ou_batch *batch, int waves, int dir_groups);
 /* use_graph = 0 launches the same kernels eagerly (for comparison). */
 int ou_batch_set_graph(ou_batch *batch, int use_graph);
+/* n_streams = 2 (the default) launches each merged level as two kernels, the
+ * members' first octant halves on the batch's stream and their second halves
+ * on a side stream forked from and joined back into it (inside the captured
+ * graph as well); 1 launches each level as one kernel.  Results are
+ * bit-for-bit the same. */
+int ou_batch_set_streams(ou_batch *batch, int n_streams);
 
 /* Apply of every member: d_y[i] = d_x[i] - D L^-1 M S d_x[i] (device vectors;
  * d_x / d_y are host arrays of n_plans device pointers; all d_y distinct and

@@ -153,6 +153,7 @@ def lib():
         L.ou_batch_info.argtypes = [_vp, _ip, _ip, _ip, ctypes.POINTER(ctypes.c_long)]
         L.ou_batch_set_shape.argtypes = [_vp, ctypes.c_int, ctypes.c_int]
         L.ou_batch_set_graph.argtypes = [_vp, ctypes.c_int]
+        L.ou_batch_set_streams.argtypes = [_vp, ctypes.c_int]
         L.ou_batch_apply_device.argtypes = [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _vp]
         L.ou_batch_sweep_device.argtypes = [_vp, ctypes.POINTER(_vp), ctypes.c_int, _vp]
         L.ou_batch_tune.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(TuneEntry), ctypes.c_int,
@@ -559,6 +560,10 @@ class Batch:
 
     def set_graph(self, on):
         _check(lib().ou_batch_set_graph(self._b, int(bool(on))))
+
+    def set_streams(self, n):
+        """Launches per merged level: 2 (octant halves on two streams) or 1."""
+        _check(lib().ou_batch_set_streams(self._b, int(n)))
 
     def apply_device(self, xs, ys, stream=None):
         _check(lib().ou_batch_apply_device(self._b, _ptr_table(xs), _ptr_table(ys), stream))

@@ -95,6 +95,11 @@ def main(argv=None):
     eager_ms = batch.time_apply(xs, ys, a.warmup, a.steps)
     batch.set_graph(True)
     log(f"[batch] eager batched launches: {eager_ms:.2f} ms")
+    batch.set_streams(1)
+    one_stream_ms = batch.time_apply(xs, ys, a.warmup, a.steps)
+    batch.set_streams(2)
+    log(f"[batch] graph replay, one launch per level (no octant-half side stream): "
+        f"{one_stream_ms:.2f} ms")
 
     # bit-for-bit spot check of two members against their own Apply
     batch.apply_device(xs, ys)
@@ -147,6 +152,7 @@ def main(argv=None):
         "sizes": list(sizes), "block": a.block, "cells": cells, "directions": specs[0].quad.n_dir,
         "precision": "fp32 sweep, fp64 moments",
         "shape": {"waves": info["waves"], "dir_groups": info["dir_groups"]},
+        "ms_graph_one_stream": one_stream_ms,
         "levels_per_batch": info["n_levels"], "chunks_per_batch": info["n_chunks"],
         "ms_graph": graph_ms, "ms_eager": eager_ms, "ms_sequential": seq_ms,
         "updates_per_s": updates / (graph_ms * 1e-3),

@@ -90,6 +90,12 @@ Batch::Batch(std::vector<RadiativeTransfer *> members, int waves, int dir_groups
         fail(OU_ERR_ARG, "a plan appears twice in the batch (members share no state)");
     }
     OU_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    OU_HIP(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking));
+    OU_HIP(hipEventCreateWithFlags(&ev_fork_, hipEventDisableTiming));
+    OU_HIP(hipEventCreateWithFlags(&ev_join_, hipEventDisableTiming));
+    bl_.side = side_;
+    bl_.ev_fork = ev_fork_;
+    bl_.ev_join = ev_join_;
     OU_HIP(hipEventCreateWithFlags(&io_done_, hipEventDisableTiming));
     OU_HIP(hipHostMalloc(reinterpret_cast<void **>(&h_io_), 2 * members_.size() * sizeof(double *)));
     OU_HIP(hipMalloc(reinterpret_cast<void **>(&d_io_), 2 * members_.size() * sizeof(double *)));
@@ -142,6 +148,11 @@ void Batch::release() {
   h_io_ = nullptr;
   if (io_done_) (void)hipEventDestroy(io_done_);
   io_done_ = nullptr;
+  for (hipEvent_t e : {ev_fork_, ev_join_})
+    if (e) (void)hipEventDestroy(e);
+  ev_fork_ = ev_join_ = nullptr;
+  if (side_) (void)hipStreamDestroy(side_);
+  side_ = nullptr;
   if (stream_) (void)hipStreamDestroy(stream_);
   stream_ = nullptr;
 }
@@ -175,13 +186,18 @@ void Batch::build_schedule() {
   std::vector<Chunk> chunks;
   std::vector<int> view;
   bl_.level_off.assign(1, 0);
+  bl_.level_split.assign(size_t(n_levels), 0);
   for (int l = 0; l < n_levels; ++l) {
-    for (size_t u = 0; u < members_.size(); ++u) {
-      const Plan &pl = members_[u]->plan();
-      if (l >= pl.n_levels) continue;
-      for (int c = pl.level_chunk_off[l]; c < pl.level_chunk_off[l + 1]; ++c) {
-        chunks.push_back(host[u][size_t(c)]);
-        view.push_back(int(u));
+    // every member's first octant half (quarters 0-1), then every second half
+    for (int half = 0; half < 2; ++half) {
+      if (half == 1) bl_.level_split[size_t(l)] = int(chunks.size());
+      for (size_t u = 0; u < members_.size(); ++u) {
+        const Plan &pl = members_[u]->plan();
+        if (l >= pl.n_levels) continue;
+        for (int c = pl.quarter_begin(l, 2 * half); c < pl.quarter_begin(l, 2 * half + 2); ++c) {
+          chunks.push_back(host[u][size_t(c)]);
+          view.push_back(int(u));
+        }
       }
     }
     bl_.level_off.push_back(int(chunks.size()));
@@ -256,6 +272,13 @@ void Batch::set_shape(int waves, int dir_groups) {
   batch_prepare(bl_);
   drop_graphs();
   build_views();
+}
+
+void Batch::set_streams(int n) {
+  if (n != 1 && n != 2) fail(OU_ERR_ARG, "batch streams must be 1 or 2");
+  OU_HIP(hipStreamSynchronize(stream_));
+  bl_.side = n == 2 ? side_ : nullptr;
+  drop_graphs();
 }
 
 // A member plan may have been reshaped (or its partial moments reallocated)

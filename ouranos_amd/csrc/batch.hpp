@@ -60,6 +60,10 @@ class Batch {
 
   void set_shape(int waves, int dir_groups);
   void set_use_graph(bool on) { use_graph_ = on; }
+  // 2: each level's second octant half on a side stream (the default); 1: one
+  // launch per level.
+  void set_streams(int n);
+  int streams() const { return bl_.side ? 2 : 1; }
   bool use_graph() const { return use_graph_; }
   // Times every supported (waves, direction groups) shape on this batch,
   // keeps the fastest and records it in the tuning cache.
@@ -81,6 +85,8 @@ class Batch {
   std::vector<RadiativeTransfer *> members_;
   BatchLaunch bl_;
   hipStream_t stream_ = nullptr;
+  hipStream_t side_ = nullptr;  // the second octant half's level launches
+  hipEvent_t ev_fork_ = nullptr, ev_join_ = nullptr;
   Chunk *d_chunks_ = nullptr;
   int *d_chunk_view_ = nullptr;
   SweepView *d_views_ = nullptr;     // [2][members]: plain, with source

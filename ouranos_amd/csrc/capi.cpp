@@ -555,6 +555,13 @@ int ou_batch_set_graph(ou_batch *batch, int use_graph) {
   });
 }
 
+int ou_batch_set_streams(ou_batch *batch, int n_streams) {
+  return guarded([&] {
+    batch_of(batch).set_streams(n_streams);
+    return int(OU_OK);
+  });
+}
+
 int ou_batch_apply_device(ou_batch *batch, const double *const *d_x, double *const *d_y,
                           void *stream) {
   return guarded([&] {

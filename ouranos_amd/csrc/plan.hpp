@@ -155,6 +155,13 @@ struct BatchLaunch {
   int n_members = 0, dim = 3, p = 1, N = 8, mm = 1, precision = OU_FP32;
   int waves = 4, dir_groups = 1;
   std::vector<int> level_off;  // host
+  // host, [n_levels]: first chunk of level l in the second octant half of
+  // every member (each merged level lists all members' first halves first)
+  std::vector<int> level_split;
+  // With side set, a level's second half is launched on it (forked from and
+  // joined back into the launch stream by the two events).
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   const Chunk *chunks = nullptr;
   const int *chunk_view = nullptr;
   const int *active = nullptr;           // [member]: 0 = skip the member's work

@@ -484,11 +484,26 @@ void launch_batch_levels_impl(const BatchLaunch &bl, const SweepView *views, hip
   }
   const unsigned groups = unsigned(bl.dir_groups);
   const int n_levels = int(bl.level_off.size()) - 1;
+  // the second octant half of every member on the side stream (see
+  // launch_levels_impl)
+  const bool split = bl.side != nullptr;
+  if (split) {
+    OU_HIP(hipEventRecord(bl.ev_fork, s));
+    OU_HIP(hipStreamWaitEvent(bl.side, bl.ev_fork, 0));
+  }
   for (int l = 0; l < n_levels; ++l) {
     const int b = bl.level_off[l], e = bl.level_off[l + 1];
-    if (e > b)
-      hipLaunchKernelGGL(kern, dim3(e - b, groups), dim3(64 * W), C::lds_bytes, s, views,
+    const int m = split ? bl.level_split[l] : e;
+    if (m > b)
+      hipLaunchKernelGGL(kern, dim3(m - b, groups), dim3(64 * W), C::lds_bytes, s, views,
                          bl.chunk_view, bl.chunks, b, bl.active);
+    if (e > m)
+      hipLaunchKernelGGL(kern, dim3(e - m, groups), dim3(64 * W), C::lds_bytes, bl.side, views,
+                         bl.chunk_view, bl.chunks, m, bl.active);
+  }
+  if (split) {
+    OU_HIP(hipEventRecord(bl.ev_join, bl.side));
+    OU_HIP(hipStreamWaitEvent(s, bl.ev_join, 0));
   }
   OU_HIP(hipGetLastError());
 }

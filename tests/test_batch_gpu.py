@@ -75,6 +75,15 @@ def test_batch_apply_equals_single_bitwise(precision, shape):
     for i, (op, x, y) in enumerate(zip(ops, xs, ys)):
         y1 = op.Apply(x)
         assert np.array_equal(y, y1), (i, rel_err(y, y1))
+    # one launch per merged level instead of two (octant halves on two streams)
+    batch.set_streams(1)
+    for y, y1 in zip(batch.apply(xs), ys):
+        assert np.array_equal(y, y1)
+    batch.set_streams(2)
+    for y, y1 in zip(batch.apply(xs), ys):
+        assert np.array_equal(y, y1)
+    with pytest.raises(ou.OuError):
+        batch.set_streams(4)
     batch.close()
 
 
