@@ -4,7 +4,9 @@
   refined   FP32 sweep with FP64 moment accumulation inside FGMRES(30),
             restarted from FP64 residuals of the FP64 operator
             (iterative refinement), no preconditioner
-  learned   as refined, right-preconditioned by the trained U-Net
+  learned   as refined, right-preconditioned by the trained U-Net, with
+            FGMRES(16): restarts at 16 keep every Gram-Schmidt pass in one
+            fused sweep and converged in fewer steps than 30 on this workload
 
 Every residual test is FP64 on the FP64 operator; the FP32 runs report the
 relative L2 difference of their scalar flux (moment 0) from the plain FP64
@@ -41,7 +43,9 @@ def main(argv=None):
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--inner-tol", type=float, default=3e-5)
     ap.add_argument("--max-it", type=int, default=3000)
-    ap.add_argument("--restart", type=int, default=30)
+    ap.add_argument("--restart", type=int, default=30,
+                    help="restart length of plain GMRES and the refined solve")
+    ap.add_argument("--learned-restart", type=int, default=16)
     ap.add_argument("--weights", default=WEIGHTS)
     ap.add_argument("--grid", type=int, default=0,
                     help="coarse grid of the network (default: the one it was trained on)")
@@ -70,7 +74,8 @@ def main(argv=None):
     op32 = spec.operator(ou.FP32)
     log(f"[solve] plans built in {time.perf_counter() - t0:.1f} s")
     nd = spec.n_dofs
-    out = {"workload": spec.name, "tol": a.tol, "restart": a.restart, "unknowns": spec.size,
+    out = {"workload": spec.name, "tol": a.tol, "restart": a.restart,
+           "learned_restart": a.learned_restart, "unknowns": spec.size,
            "device": ou.device_name()}
     sols = {}
 
@@ -94,7 +99,8 @@ def main(argv=None):
         if name in skip:
             continue
         phi, info = ou.solve_refined(op64, op32, n, tol=a.tol, max_it=a.max_it,
-                                     restart=a.restart, inner_tol=a.inner_tol,
+                                     restart=a.learned_restart if n is not None else a.restart,
+                                     inner_tol=a.inner_tol,
                                      verbose=a.verbose, raise_on_fail=False)
         if "plain" in sols:
             ref = sols["plain"]
