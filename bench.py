@@ -67,7 +67,7 @@ def cpu_baseline(spec, seconds_budget, threads):
     O.lib()
     n = list(spec.n)
     # slab size chosen so one direction takes roughly seconds_budget/2
-    per_cell_s = 10e-6 * (spec.n_loc / 27.0) ** 2.2
+    per_cell_s = 5e-6 * (spec.n_loc / 27.0) ** 2.2  # measured 5.2 us per Q2 cell-direction
     layers = max(1, min(n[-1], int(seconds_budget / 2 / per_cell_s / (n[0] * n[1]))))
     sub_n = n[:-1] + [layers] if spec.dim == 3 else n[:-1] + [layers]
     nc = int(np.prod(sub_n))
