@@ -34,7 +34,15 @@ typedef enum {
 } ou_status;
 
 typedef enum { OU_FP64 = 0, OU_FP32 = 1 } ou_precision;
-typedef enum { OU_BC_VACUUM = 0, OU_BC_ISOTROPIC = 1, OU_BC_MOST_NORMAL = 2 } ou_bc;
+/* OU_BC_REFLECTIVE is what a parameter file may name (Parameters.cc:59-70);
+ * the reference never implements it in the sweep, and plans reject it with
+ * OU_ERR_UNSUPPORTED. */
+typedef enum {
+  OU_BC_VACUUM = 0,
+  OU_BC_ISOTROPIC = 1,
+  OU_BC_MOST_NORMAL = 2,
+  OU_BC_REFLECTIVE = 3
+} ou_bc;
 typedef enum { OU_QUAD_LS = 0, OU_QUAD_GLC = 1, OU_QUAD_PRODUCT_GLC = 2 } ou_quad_kind;
 typedef enum {
   OU_SOLVER_SI = 0,
@@ -408,6 +416,92 @@ typedef struct {
  * group.  Inputs are resident in HBM before the first event. */
 int ou_time_apply(ou_plan *plan, const double *d_x, double *d_y, int warmup,
                   int steps, ou_timing *timing);
+
+/* ------------------------------------------------------------------------ */
+/* Input files                                                              */
+/* Readers of the reference's parameter, geometry and cross-section files,  */
+/* a subset of deal.II's ParameterHandler format (`set name = value`,       */
+/* `subsection name` ... `end`, `#` comments, a trailing backslash          */
+/* continues a line).  They replace Parameters (src/Parameters.cc:10-302),  */
+/* the reading half of Geometry (src/Geometry.cc:10-128) and                */
+/* RTMaterialProperties (src/RTMaterialProperties.cc:10-81), with the same  */
+/* entry names, defaults and patterns.  A missing file, an undeclared entry */
+/* or a value outside its pattern is OU_ERR_ARG, with the file and line in  */
+/* ou_last_error() (the reference throws "Invalid input parameters file."). */
+/* ------------------------------------------------------------------------ */
+typedef struct ou_parameters ou_parameters;
+typedef struct ou_geometry ou_geometry;
+typedef struct ou_material_properties ou_material_properties;
+
+/* Parameters::get_* (src/Parameters.hh:40-104) */
+typedef struct {
+  int dim;                  /* get_dimension */
+  int fe_order;             /* get_fe_order */
+  int n_refinements;        /* get_n_refinements (adaptive cycles) */
+  double refinement_factor; /* get_refinement_factor */
+  double coarsening_factor; /* get_coarsening_factor */
+  int solver_type;          /* OU_SOLVER_SI, OU_SOLVER_GMRES or OU_SOLVER_BICGSTAB */
+  int max_outer_it;         /* get_max_outer_it */
+  int max_inner_it;         /* get_max_inner_it */
+  double outer_tol;         /* get_outer_tolerance */
+  double inner_tol;         /* get_inner_tolerance */
+  int sn_order;             /* get_sn_order */
+  int quad_type;            /* OU_QUAD_LS or OU_QUAD_GLC (get_quad_type) */
+  double weight_sum;        /* get_weight_sum: 1, 2 pi or 4 pi */
+  int galerkin;             /* get_galerkin */
+  int bc_type[6];           /* ou_bc per face -x,+x,-y,+y,-z,+z (vacuum past 2*dim) */
+  int n_groups;             /* get_n_groups */
+  int n_src;                /* get_n_src */
+} ou_parameters_info;
+
+int ou_parameters_read(const char *path, ou_parameters **prm);
+void ou_parameters_destroy(ou_parameters *prm);
+int ou_parameters_get_info(const ou_parameters *prm, ou_parameters_info *info);
+/* get_geometry_filename (which = 0), get_xs_filename (1), get_output_filename
+ * (2): copies at most len-1 characters and a terminating zero into buf (buf
+ * may be NULL); *needed = the name's length + 1. */
+int ou_parameters_get_filename(const ou_parameters *prm, int which, char *buf, int len,
+                               int *needed);
+/* get_inc_flux(face, group) and get_src(source, group) */
+int ou_parameters_get_inc_flux(const ou_parameters *prm, int face, int group, double *value);
+int ou_parameters_get_src(const ou_parameters *prm, int source, int group, double *value);
+
+/* Geometry's reader: a subdivided hyper-rectangle with a material and a
+ * source id per coarse cell, globally refined n_global_refinements times
+ * (every child keeps its parent's ids). */
+typedef struct {
+  int dim;
+  double bottom_left[3];    /* Bottom left point */
+  double upper_right[3];    /* Upper right point */
+  int n_subdivisions[3];    /* get_n_subdivisions */
+  int n_global_refinements;
+  int n_materials;          /* get_n_materials: largest material id + 1 */
+  int n_cells[3];           /* cells per axis after the refinements (1 past dim) */
+  double cell_size[3];
+} ou_geometry_info;
+
+int ou_geometry_read(const char *path, int dim, ou_geometry **geo);
+void ou_geometry_destroy(ou_geometry *geo);
+int ou_geometry_get_info(const ou_geometry *geo, ou_geometry_info *info);
+/* get_material_ids(i, j[, k]) and get_source_ids(i, j[, k]) of coarse cell
+ * (i, j, k) (k = 0 in 2D; src/Geometry.hh:45-52); either output may be NULL. */
+int ou_geometry_get_ids(const ou_geometry *geo, int i, int j, int k, int *material_id,
+                        int *source_id);
+/* Ids of every cell of the refined mesh, x fastest (the plans' cell order):
+ * n_cells[0] * n_cells[1] * n_cells[2] values each; either output may be NULL. */
+int ou_geometry_cell_ids(const ou_geometry *geo, int *material_id, int *source_id);
+
+/* RTMaterialProperties(file, n_materials, n_groups) and its getters
+ * (src/RTMaterialProperties.hh:31-46, layout :84-101). */
+int ou_material_properties_read(const char *path, int n_materials, int n_groups,
+                                ou_material_properties **mp);
+void ou_material_properties_destroy(ou_material_properties *mp);
+int ou_material_properties_get_L_max(const ou_material_properties *mp, int *L_max);
+int ou_material_properties_get_sigma_t(const ou_material_properties *mp, int mat, int group,
+                                       double *value);
+/* get_sigma_s(mat, group, group_to, l) */
+int ou_material_properties_get_sigma_s(const ou_material_properties *mp, int mat, int group,
+                                       int group_to, int l, double *value);
 
 #ifdef __cplusplus
 }

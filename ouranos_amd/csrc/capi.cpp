@@ -1,12 +1,14 @@
 // extern "C" entry points of include/ouranos_amd.h.  No exception crosses
 // this boundary: every call returns OU_OK or a negative ou_status and keeps
 // the message for ou_last_error().
+#include <algorithm>
 #include <chrono>
 #include <cstring>
 #include <string>
 
 #include "batch.hpp"
 #include "fe.hpp"
+#include "inp.hpp"
 #include "learned.hpp"
 #include "quadrature.hpp"
 #include "radiative_transfer.hpp"
@@ -1006,6 +1008,223 @@ int ou_solve_refined(ou_plan *plan64, ou_plan *plan32, ou_unet *net, const ou_so
     OU_HIP(hipMemcpy(phi, d_phi, bytes, hipMemcpyDeviceToHost));
     fill_info(info, r.total, secs, r.outer, prec ? prec->applications() : 0);
     return r.total.converged ? int(OU_OK) : int(OU_ERR_NOT_CONVERGED);
+  });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Input files
+// ---------------------------------------------------------------------------
+struct ou_parameters {
+  ou::Parameters p;
+};
+struct ou_geometry {
+  ou::Geometry g;
+};
+struct ou_material_properties {
+  ou::MaterialProperties m;
+};
+
+namespace {
+void check_index(int i, int n, const char *what) {
+  if (i < 0 || i >= n) ou::fail(OU_ERR_ARG, std::string(what) + " index out of range");
+}
+}  // namespace
+
+extern "C" {
+
+int ou_parameters_read(const char *path, ou_parameters **prm) {
+  return guarded([&] {
+    if (!path || !prm) ou::fail(OU_ERR_ARG, "null argument");
+    *prm = nullptr;
+    *prm = new ou_parameters{ou::Parameters(path)};
+    return int(OU_OK);
+  });
+}
+
+void ou_parameters_destroy(ou_parameters *prm) { delete prm; }
+
+int ou_parameters_get_info(const ou_parameters *prm, ou_parameters_info *info) {
+  return guarded([&] {
+    if (!prm || !info) ou::fail(OU_ERR_ARG, "null argument");
+    const ou::Parameters &p = prm->p;
+    ou_parameters_info r{};
+    r.dim = p.dim;
+    r.fe_order = p.fe_order;
+    r.n_refinements = p.n_refinements;
+    r.refinement_factor = p.refinement_factor;
+    r.coarsening_factor = p.coarsening_factor;
+    r.solver_type = p.solver_type == ou::INP_SI ? OU_SOLVER_SI
+                    : p.solver_type == ou::INP_BICGSTAB ? OU_SOLVER_BICGSTAB
+                                                          : OU_SOLVER_GMRES;
+    r.max_outer_it = p.max_out_it;
+    r.max_inner_it = p.max_in_it;
+    r.outer_tol = p.outer_tol;
+    r.inner_tol = p.inner_tol;
+    r.sn_order = p.sn_order;
+    r.quad_type = p.quad_type == ou::INP_LS ? OU_QUAD_LS : OU_QUAD_GLC;
+    r.weight_sum = p.weight_sum;
+    r.galerkin = p.galerkin ? 1 : 0;
+    for (int f = 0; f < 6; ++f) {
+      int bc = OU_BC_VACUUM;
+      if (f < 2 * p.dim) {
+        const int b = p.bc_type[size_t(f)];
+        bc = b == ou::INP_ISOTROPIC     ? OU_BC_ISOTROPIC
+             : b == ou::INP_MOST_NORMAL ? OU_BC_MOST_NORMAL
+             : b == ou::INP_REFLECTIVE  ? OU_BC_REFLECTIVE
+                                        : OU_BC_VACUUM;
+      }
+      r.bc_type[f] = bc;
+    }
+    r.n_groups = p.n_groups;
+    r.n_src = p.n_src;
+    *info = r;
+    return int(OU_OK);
+  });
+}
+
+int ou_parameters_get_filename(const ou_parameters *prm, int which, char *buf, int len,
+                               int *needed) {
+  return guarded([&] {
+    if (!prm) ou::fail(OU_ERR_ARG, "null parameters");
+    const ou::Parameters &p = prm->p;
+    const std::string *s = which == 0   ? &p.geometry_filename
+                           : which == 1 ? &p.xs_filename
+                           : which == 2 ? &p.output_filename
+                                        : nullptr;
+    if (!s) ou::fail(OU_ERR_ARG, "file name selector must be 0, 1 or 2");
+    if (needed) *needed = int(s->size()) + 1;
+    if (buf && len > 0) {
+      const size_t n = std::min(s->size(), size_t(len - 1));
+      std::memcpy(buf, s->data(), n);
+      buf[n] = '\0';
+    }
+    return int(OU_OK);
+  });
+}
+
+int ou_parameters_get_inc_flux(const ou_parameters *prm, int face, int group, double *value) {
+  return guarded([&] {
+    if (!prm || !value) ou::fail(OU_ERR_ARG, "null argument");
+    check_index(face, 2 * prm->p.dim, "face");
+    check_index(group, prm->p.n_groups, "group");
+    *value = prm->p.inc_flux[size_t(face)][size_t(group)];
+    return int(OU_OK);
+  });
+}
+
+int ou_parameters_get_src(const ou_parameters *prm, int source, int group, double *value) {
+  return guarded([&] {
+    if (!prm || !value) ou::fail(OU_ERR_ARG, "null argument");
+    check_index(source, prm->p.n_src, "source");
+    check_index(group, prm->p.n_groups, "group");
+    *value = prm->p.src[size_t(source)][size_t(group)];
+    return int(OU_OK);
+  });
+}
+
+int ou_geometry_read(const char *path, int dim, ou_geometry **geo) {
+  return guarded([&] {
+    if (!path || !geo) ou::fail(OU_ERR_ARG, "null argument");
+    *geo = nullptr;
+    *geo = new ou_geometry{ou::Geometry(path, dim)};
+    return int(OU_OK);
+  });
+}
+
+void ou_geometry_destroy(ou_geometry *geo) { delete geo; }
+
+int ou_geometry_get_info(const ou_geometry *geo, ou_geometry_info *info) {
+  return guarded([&] {
+    if (!geo || !info) ou::fail(OU_ERR_ARG, "null argument");
+    const ou::Geometry &g = geo->g;
+    ou_geometry_info r{};
+    r.dim = g.dim;
+    const std::vector<int> n = g.n_cells();
+    const std::vector<double> h = g.cell_size();
+    for (int a = 0; a < 3; ++a) {
+      r.bottom_left[a] = a < g.dim ? g.bottom_left[size_t(a)] : 0.0;
+      r.upper_right[a] = a < g.dim ? g.upper_right[size_t(a)] : 0.0;
+      r.n_subdivisions[a] = a < g.dim ? g.n_subdivisions[size_t(a)] : 1;
+      r.n_cells[a] = n[size_t(a)];
+      r.cell_size[a] = h[size_t(a)];
+    }
+    r.n_global_refinements = g.n_global_refinements;
+    r.n_materials = g.n_materials;
+    *info = r;
+    return int(OU_OK);
+  });
+}
+
+int ou_geometry_get_ids(const ou_geometry *geo, int i, int j, int k, int *material_id,
+                        int *source_id) {
+  return guarded([&] {
+    if (!geo) ou::fail(OU_ERR_ARG, "null geometry");
+    const ou::Geometry &g = geo->g;
+    check_index(i, g.n_subdivisions[0], "i");
+    check_index(j, g.n_subdivisions[1], "j");
+    check_index(k, g.dim == 3 ? g.n_subdivisions[2] : 1, "k");
+    const size_t c = size_t(i) + size_t(g.n_subdivisions[0]) *
+                                     (size_t(j) + size_t(g.n_subdivisions[1]) * size_t(k));
+    if (material_id) *material_id = g.material_ids[c];
+    if (source_id) *source_id = g.source_ids[c];
+    return int(OU_OK);
+  });
+}
+
+int ou_geometry_cell_ids(const ou_geometry *geo, int *material_id, int *source_id) {
+  return guarded([&] {
+    if (!geo) ou::fail(OU_ERR_ARG, "null geometry");
+    std::vector<int> m, s;
+    geo->g.fine_ids(m, s);
+    if (material_id) std::copy(m.begin(), m.end(), material_id);
+    if (source_id) std::copy(s.begin(), s.end(), source_id);
+    return int(OU_OK);
+  });
+}
+
+int ou_material_properties_read(const char *path, int n_materials, int n_groups,
+                                ou_material_properties **mp) {
+  return guarded([&] {
+    if (!path || !mp) ou::fail(OU_ERR_ARG, "null argument");
+    *mp = nullptr;
+    *mp = new ou_material_properties{ou::MaterialProperties(path, n_materials, n_groups)};
+    return int(OU_OK);
+  });
+}
+
+void ou_material_properties_destroy(ou_material_properties *mp) { delete mp; }
+
+int ou_material_properties_get_L_max(const ou_material_properties *mp, int *L_max) {
+  return guarded([&] {
+    if (!mp || !L_max) ou::fail(OU_ERR_ARG, "null argument");
+    *L_max = mp->m.L_max;
+    return int(OU_OK);
+  });
+}
+
+int ou_material_properties_get_sigma_t(const ou_material_properties *mp, int mat, int group,
+                                       double *value) {
+  return guarded([&] {
+    if (!mp || !value) ou::fail(OU_ERR_ARG, "null argument");
+    check_index(mat, mp->m.n_materials, "material");
+    check_index(group, mp->m.n_groups, "group");
+    *value = mp->m.get_sigma_t(mat, group);
+    return int(OU_OK);
+  });
+}
+
+int ou_material_properties_get_sigma_s(const ou_material_properties *mp, int mat, int group,
+                                       int group_to, int l, double *value) {
+  return guarded([&] {
+    if (!mp || !value) ou::fail(OU_ERR_ARG, "null argument");
+    check_index(mat, mp->m.n_materials, "material");
+    check_index(group, mp->m.n_groups, "group");
+    check_index(group_to, mp->m.n_groups, "group");
+    check_index(l, mp->m.L_max + 1, "Legendre order");
+    *value = mp->m.get_sigma_s(mat, group, group_to, l);
+    return int(OU_OK);
   });
 }
 

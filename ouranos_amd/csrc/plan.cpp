@@ -138,8 +138,11 @@ Plan *create_plan(const ou_problem_desc *desc) {
     for (int f = 0; f < 6; ++f) {
       p.bc[f] = f < 2 * p.dim ? ds.bc[f] : OU_BC_VACUUM;
       p.inc[f] = f < 2 * p.dim ? ds.inc_flux[f] : 0.0;
+      if (p.bc[f] == OU_BC_REFLECTIVE)
+        fail(OU_ERR_UNSUPPORTED, "reflective boundaries are not implemented (the reference "
+                                 "parses them but never applies them)");
       if (p.bc[f] < OU_BC_VACUUM || p.bc[f] > OU_BC_MOST_NORMAL)
-        fail(OU_ERR_UNSUPPORTED, "boundary condition must be vacuum, isotropic or most normal");
+        fail(OU_ERR_ARG, "unknown boundary condition");
       if (p.bc[f] == OU_BC_MOST_NORMAL && !ds.most_normal)
         fail(OU_ERR_ARG, "most-normal boundary needs the most_normal flags");
     }
