@@ -503,6 +503,17 @@ int ou_material_properties_get_sigma_t(const ou_material_properties *mp, int mat
 int ou_material_properties_get_sigma_s(const ou_material_properties *mp, int mat, int group,
                                        int group_to, int l, double *value);
 
+/* output_results (src/ouranos.cc:211-253): legacy VTK of the flux moments.
+ * Writes `<filename>0000.vtk` (one VTK cell per mesh cell with its own 2^dim
+ * corner points; point fields flux_<ggg>g_<mmm>m with the corner values of
+ * every group and moment; cell field partitioning = 0) and the VisIt record
+ * `<filename>.vtk` naming it.  flux is [group][moment][dof] on the mesh of
+ * n_cells cells of size cell_size from origin, cells x fastest and each
+ * cell's (fe_order+1)^dim dofs lexicographic (the plans' layout). */
+int ou_write_vtk(const char *filename, int dim, int fe_order, const int *n_cells,
+                 const double *cell_size, const double *origin, int n_groups, int n_mom,
+                 const double *flux);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1016,6 +1016,12 @@ int ou_solve_refined(ou_plan *plan64, ou_plan *plan32, ou_unet *net, const ou_so
 // ---------------------------------------------------------------------------
 // Input files
 // ---------------------------------------------------------------------------
+namespace ou {
+void write_vtk(const std::string &filename, int dim, int p, const int *n_cells,
+               const double *cell_size, const double *origin, int n_groups, int n_mom,
+               const double *flux);
+}
+
 struct ou_parameters {
   ou::Parameters p;
 };
@@ -1224,6 +1230,17 @@ int ou_material_properties_get_sigma_s(const ou_material_properties *mp, int mat
     check_index(group_to, mp->m.n_groups, "group");
     check_index(l, mp->m.L_max + 1, "Legendre order");
     *value = mp->m.get_sigma_s(mat, group, group_to, l);
+    return int(OU_OK);
+  });
+}
+
+int ou_write_vtk(const char *filename, int dim, int fe_order, const int *n_cells,
+                 const double *cell_size, const double *origin, int n_groups, int n_mom,
+                 const double *flux) {
+  return guarded([&] {
+    if (!filename || !n_cells || !cell_size || !origin || !flux)
+      ou::fail(OU_ERR_ARG, "null argument");
+    ou::write_vtk(filename, dim, fe_order, n_cells, cell_size, origin, n_groups, n_mom, flux);
     return int(OU_OK);
   });
 }

@@ -10,11 +10,14 @@ selects, as ouranos.cc:43-209 does for one energy group:
 * GMRES / BICGSTAB: the Krylov solve of (I - D L^-1 M Sigma) phi = D L^-1 q
   with "Max inner iterations" and the inner tolerance (ouranos.cc:131-171;
   GMRES restarts at 30, deal.II's default).
-Writes the flux moments, moment-major [m][dof] with cells numbered x fastest
-and each cell's dofs lexicographically, to `<Output file>.npy` (the reference
-writes VTK files instead), and prints one JSON line with the solve's outcome.
+Writes the flux moments as the reference's output_results does (ouranos.cc:
+211-253): `<Output file>0000.vtk` with the point fields flux_000g_<mmm>m and
+the record `<Output file>.vtk`.  Also writes them, moment-major [m][dof] with
+cells numbered x fastest and each cell's dofs lexicographically, to
+`<Output file>.npy`, and prints one JSON line with the solve's outcome.
 """
 import argparse
+import ctypes
 import json
 import sys
 import time
@@ -22,14 +25,33 @@ import time
 import numpy as np
 
 import ouranos_amd as ou
-from ouranos_amd.inp import problem_from_parameters
+from ouranos_amd.inp import Geometry, problem_from_parameters, resolve
 
 SOLVER_NAMES = {ou.SI: "SI", ou.GMRES: "GMRES", ou.BICGSTAB: "BICGSTAB"}
+
+
+def output_results(filename, spec, group_flux, origin=None):
+    """Legacy VTK of the flux moments (ou_write_vtk): group_flux is one
+    moment vector per group ([m][dof] each) on the mesh of `spec`."""
+    L = ou.lib()
+    dp, ip = ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)
+    L.ou_write_vtk.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ip, dp, dp,
+                               ctypes.c_int, ctypes.c_int, dp]
+    flux = np.ascontiguousarray(np.stack([np.asarray(g, dtype=np.float64)
+                                          for g in group_flux]))
+    n = (ctypes.c_int * 3)(*(list(spec.n) + [1] * (3 - len(spec.n))))
+    h = (ctypes.c_double * 3)(*(list(spec.h) + [1.0] * (3 - len(spec.h))))
+    o = (ctypes.c_double * 3)(*((list(origin) if origin is not None else [0.0] * spec.dim)
+                                + [0.0] * (3 - spec.dim)))
+    ou._check(L.ou_write_vtk(str(filename).encode(), spec.dim, spec.p, n, h, o, len(flux),
+                             spec.quad.n_mom, flux.ctypes.data_as(dp)))
+    return str(filename) + "0000.vtk"
 
 
 def run(parameters_path, precision=ou.FP64, verbose=False, output=None):
     """Solves the problem of one parameter file.  Returns (phi, summary)."""
     prm, spec = problem_from_parameters(parameters_path)
+    geo = Geometry(resolve(prm.get_geometry_filename(), parameters_path), spec.dim)
     op = spec.operator(precision)
     t0 = time.perf_counter()
     phi, info = op.solve(solver=prm.get_solver_type(), tol=prm.get_inner_tolerance(),
@@ -44,8 +66,9 @@ def run(parameters_path, precision=ou.FP64, verbose=False, output=None):
                "converged": info["converged"], "seconds": seconds}
     out = output if output is not None else prm.get_output_filename()
     if out:
-        np.save(out if out.endswith(".npy") else out + ".npy", phi)
-        summary["output"] = out if out.endswith(".npy") else out + ".npy"
+        np.save(out + ".npy", phi)
+        summary["output"] = out + ".npy"
+        summary["vtk"] = output_results(out, spec, [phi], origin=geo.get_bottom_left())
     return phi, summary
 
 

@@ -36,6 +36,7 @@ def test_reference_parameter_files_solve_to_the_golden_values(tmp_path, solver):
     assert np.abs(mine - gold).max() < 1e-5
     saved = np.load(summary["output"])
     assert np.array_equal(saved, phi)
+    assert os.path.getsize(summary["vtk"]) > 0 and os.path.exists(str(tmp_path / "phi.vtk"))
 
 
 def test_parameter_file_problem_equals_the_built_in_problem():
