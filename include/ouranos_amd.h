@@ -503,6 +503,28 @@ int ou_material_properties_get_sigma_t(const ou_material_properties *mp, int mat
 int ou_material_properties_get_sigma_s(const ou_material_properties *mp, int mat, int group,
                                        int group_to, int l, double *value);
 
+/* ------------------------------------------------------------------------ */
+/* Several energy groups (src/ouranos.cc:62-205): one plan per group (its    */
+/* sigma_t, sigma_s(g->g), source and inflow), coupled by the scattering     */
+/* from the other groups.                                                    */
+/* ------------------------------------------------------------------------ */
+/* compute_outer_scattering_source (src/RadiativeTransfer.cc:845-876), one
+ * other group at a time: S_m = sigma_s[mat][l(m)] M x_m with the caller's
+ * table sigma_s (n_mat x (L+1), e.g. sigma_s(g'->g) of each material), added
+ * to the stored scattering source when accumulate != 0.  x is group g''s
+ * moments (host, n_mom * n_dofs).  A following ou_sweep(plan, y, 1) sweeps
+ * the stored source with the group's own source and inflow: with every other
+ * group accumulated, y = D L^-1 (sum_g' S_g'->g + M q_g / ws + inflow_g), the
+ * right-hand side of group g.  (The reference uses the moment index as the
+ * Legendre order here too; this uses l(m), defect A.5.) */
+int ou_scatter_source_with(ou_plan *plan, const double *x, const double *sigma_s,
+                           int accumulate);
+/* Solves A phi = rhs for the plan's operator A with the solver of opts
+ * (SI: phi <- phi - A phi + rhs, stopping on the relative change; GMRES /
+ * BICGSTAB on the residual), from the initial guess in phi (host vectors). */
+int ou_solve_rhs(ou_plan *plan, const ou_solver_opts *opts, const double *rhs, double *phi,
+                 ou_solve_info *info);
+
 /* output_results (src/ouranos.cc:211-253): legacy VTK of the flux moments.
  * Writes `<filename>0000.vtk` (one VTK cell per mesh cell with its own 2^dim
  * corner points; point fields flux_<ggg>g_<mmm>m with the corner values of

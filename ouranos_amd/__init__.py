@@ -389,6 +389,26 @@ class RadiativeTransfer:
         _check(lib().ou_sweep(self._plan, _d(y), int(bool(with_source))))
         return y
 
+    def scatter_source_with(self, x, sigma_s, accumulate=False):
+        """Stored scattering source (+)= sigma_s[mat][l(m)] M x_m with another
+        group's table sigma_s (n_mat x (L+1)): ou_scatter_source_with."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        sig = np.ascontiguousarray(sigma_s, dtype=np.float64)
+        _check(lib().ou_scatter_source_with(self._plan, _d(x), _d(sig), int(bool(accumulate))))
+
+    def solve_rhs(self, rhs, solver=GMRES, tol=1e-8, max_it=1000, restart=30, phi0=None,
+                  verbose=False, raise_on_fail=True):
+        """Solves A phi = rhs (ou_solve_rhs) from phi0 (zero by default)."""
+        rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+        phi = np.zeros(self.n) if phi0 is None else np.array(phi0, dtype=np.float64)
+        opts = SolverOpts(int(solver), float(tol), int(max_it), int(restart), int(verbose))
+        info = SolveInfo()
+        code = lib().ou_solve_rhs(self._plan, ctypes.byref(opts), _d(rhs), _d(phi),
+                                  ctypes.byref(info))
+        if code != 0 and (raise_on_fail or code != -6):
+            _check(code)
+        return phi, info.as_dict()
+
     # --- drivers ---------------------------------------------------------------
     def solve(self, solver=GMRES, tol=1e-8, max_it=1000, restart=30, phi0=None,
               verbose=False, raise_on_fail=True):

@@ -366,6 +366,84 @@ int ou_solve(ou_plan *plan, const ou_solver_opts *opts, double *phi, ou_solve_in
   });
 }
 
+int ou_scatter_source_with(ou_plan *plan, const double *x, const double *sigma_s,
+                           int accumulate) {
+  return guarded([&] {
+    ou::Plan &pl = plan_of(plan);
+    if (!x || !sigma_s) ou::fail(OU_ERR_ARG, "null argument");
+    check_finite_host(x, pl.vec_size);
+    check_finite_host(sigma_s, long(pl.sigma_s.size()));
+    ensure_staging(pl);
+    if (!pl.d_sigma_other) {
+      void *p = nullptr;
+      OU_HIP(hipMalloc(&p, pl.sigma_s.size() * 8));
+      pl.d_sigma_other = static_cast<double *>(p);
+    }
+    OU_HIP(hipMemcpyAsync(pl.d_sigma_other, sigma_s, pl.sigma_s.size() * 8,
+                          hipMemcpyHostToDevice, pl.stream));
+    OU_HIP(hipMemcpyAsync(pl.d_x, x, size_t(pl.vec_size) * 8, hipMemcpyHostToDevice,
+                          pl.stream));
+    ou::launch_scatter_source_with(pl, pl.d_x, pl.d_sigma_other, accumulate != 0, pl.stream);
+    OU_HIP(hipStreamSynchronize(pl.stream));
+    return int(OU_OK);
+  });
+}
+
+int ou_solve_rhs(ou_plan *plan, const ou_solver_opts *opts, const double *rhs, double *phi,
+                 ou_solve_info *info) {
+  return guarded([&] {
+    ou::Plan &pl = plan_of(plan);
+    if (!opts || !rhs || !phi) ou::fail(OU_ERR_ARG, "null argument");
+    if (!(opts->tol > 0.0) || opts->max_it < 1) ou::fail(OU_ERR_ARG, "bad tolerance or max_it");
+    check_finite_host(rhs, pl.vec_size);
+    check_finite_host(phi, pl.vec_size);
+    ou::RadiativeTransfer &rt = *plan->rt;
+    const size_t bytes = size_t(pl.vec_size) * 8;
+    double *d_phi = nullptr, *d_rhs = nullptr;
+    struct Free {
+      double **a, **b;
+      ~Free() {
+        if (*a) (void)hipFree(*a);
+        if (*b) (void)hipFree(*b);
+      }
+    } guard{&d_phi, &d_rhs};
+    OU_HIP(hipMalloc(&d_phi, bytes));
+    OU_HIP(hipMalloc(&d_rhs, bytes));
+    OU_HIP(hipMemcpy(d_phi, phi, bytes, hipMemcpyHostToDevice));
+    OU_HIP(hipMemcpy(d_rhs, rhs, bytes, hipMemcpyHostToDevice));
+    const auto t0 = std::chrono::steady_clock::now();
+    hipStream_t s = pl.stream;
+    ou::SolveResult r;
+    switch (opts->solver) {
+      case OU_SOLVER_SI:
+        r = ou::solve_si_rhs(rt, d_rhs, d_phi, opts->tol, opts->max_it, opts->verbose != 0, s);
+        break;
+      case OU_SOLVER_BICGSTAB:
+        r = ou::solve_bicgstab(rt, d_rhs, d_phi, opts->tol, opts->max_it, opts->verbose != 0, s);
+        break;
+      case OU_SOLVER_GMRES:
+      case OU_SOLVER_FGMRES:
+        r = ou::solve_gmres(rt, d_rhs, d_phi, opts->restart, opts->tol, opts->max_it, nullptr,
+                            opts->verbose != 0, s);
+        break;
+      default:
+        ou::fail(OU_ERR_ARG, "unknown solver");
+    }
+    const double secs =
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    OU_HIP(hipMemcpy(phi, d_phi, bytes, hipMemcpyDeviceToHost));
+    if (info) {
+      info->iterations = r.iterations;
+      info->rel_residual = r.rel_residual;
+      info->seconds = secs;
+      info->converged = r.converged ? 1 : 0;
+      info->fp64_applications = pl.precision == OU_FP64 ? r.iterations : 0;
+      info->preconditioner_applications = 0;
+    }
+    return r.converged ? int(OU_OK) : int(OU_ERR_NOT_CONVERGED);
+  });
+}
+
 int ou_device_alloc(size_t bytes, void **ptr) {
   return guarded([&] {
     if (!ptr) ou::fail(OU_ERR_ARG, "null output pointer");

@@ -69,7 +69,7 @@ Plan::~Plan() {
                   d_slot_d2m,  d_partial,  d_trace[0], d_trace[1], d_trace[2], d_oct_dirs,
                   d_oct_off,   d_chunks,   d_cells,    d_mat,      d_mom_l,    d_sigma_s,
                   d_source,    d_mass,     d_rowsum,   d_inflow,   d_x,        d_y,
-                  d_slot_cls};
+                  d_slot_cls, d_sigma_other};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   for (hipEvent_t e : ev)

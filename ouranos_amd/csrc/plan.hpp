@@ -120,6 +120,7 @@ struct Plan {
   Chunk *d_chunks = nullptr;
   int *d_cells = nullptr, *d_mat = nullptr, *d_mom_l = nullptr;
   double *d_sigma_s = nullptr, *d_source = nullptr, *d_mass = nullptr;
+  double *d_sigma_other = nullptr;  // a caller's sigma_s table (ou_scatter_source_with)
   double *d_rowsum = nullptr;
   double *d_inflow = nullptr;  // [dir][3] inflow value on the upwind face of each axis
   double *d_x = nullptr, *d_y = nullptr;  // staging for the host-pointer API
@@ -181,6 +182,10 @@ void set_sweep_shape(Plan &pl, int waves, int dir_groups);
 // Launch sequences (stream-ordered, no host synchronisation).
 void launch_build_inverses(Plan &pl);
 void launch_scatter_source(Plan &pl, const double *d_x, hipStream_t s);
+// S_m (+)= sigma[mat][l(m)] M x_m with a caller's table (n_mat x (L+1), on the
+// device): a scattering source from another group's cross sections.
+void launch_scatter_source_with(Plan &pl, const double *d_x, const double *d_sigma,
+                                bool accumulate, hipStream_t s);
 void launch_source_moment0(Plan &pl, hipStream_t s);
 void launch_trace_init(Plan &pl, bool with_source, hipStream_t s);
 void launch_sweep_levels(Plan &pl, const void *S0, hipStream_t s,

@@ -94,5 +94,9 @@ SolveResult solve_gmres(Operator &op, const double *d_b, double *d_x, int restar
                         hipStream_t s, ArnoldiObserver *observer = nullptr);
 SolveResult solve_bicgstab(Operator &op, const double *d_b, double *d_x, double tol,
                            int max_it, bool verbose, hipStream_t s);
+// Source iteration on A x = b: x <- x - A x + b (= D L^-1 (M Sigma x + the
+// sources b was swept from)), stopping when |x_new - x| / |x_new| < tol.
+SolveResult solve_si_rhs(Operator &op, const double *d_b, double *d_x, double tol, int max_it,
+                         bool verbose, hipStream_t s);
 
 }  // namespace ou

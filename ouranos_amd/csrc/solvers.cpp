@@ -92,6 +92,32 @@ SolveResult solve_si(RadiativeTransfer &rt, double *d_phi, double tol, int max_i
   return res;
 }
 
+SolveResult solve_si_rhs(Operator &op, const double *d_b, double *d_x, double tol, int max_it,
+                         bool verbose, hipStream_t s) {
+  const long n = op.DomainSize();
+  DevBuf ax(n), nw(n);
+  Dotter dt(1);
+  SolveResult res;
+  for (int it = 0; it < max_it; ++it) {
+    op.Apply(d_x, ax.p, s);
+    vec_lincomb(n, 1.0, d_x, -1.0, ax.p, nw.p, s);  // x - A x = D L^-1 M Sigma x
+    vec_axpby(n, 1.0, d_b, 1.0, nw.p, s);           // + b
+    vec_axpby(n, 1.0, nw.p, -1.0, d_x, s);          // d_x <- new - old
+    const double num = dt.norm(n, d_x, s);
+    const double den = dt.norm(n, nw.p, s);
+    vec_copy(n, nw.p, d_x, s);
+    res.iterations = it + 1;
+    res.rel_residual = den > 0.0 ? num / den : 0.0;
+    if (verbose) progress("SI iteration %d: %.3e\n", it, res.rel_residual);
+    if (res.rel_residual < tol) {
+      res.converged = true;
+      break;
+    }
+  }
+  OU_HIP(hipStreamSynchronize(s));
+  return res;
+}
+
 SolveResult solve_gmres(Operator &op, const double *d_b, double *d_x, int restart,
                         double tol, int max_it, Preconditioner *prec, bool verbose,
                         hipStream_t s, ArnoldiObserver *observer) {

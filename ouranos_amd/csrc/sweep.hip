@@ -144,7 +144,7 @@ __device__ __forceinline__ void scatter_one(long gid, long n_cells, long n_dofs,
                                             const int *__restrict__ mat,
                                             const double *__restrict__ sigma_s, int L,
                                             const int *__restrict__ mom_l, const double *Mm,
-                                            T *__restrict__ S) {
+                                            T *__restrict__ S, bool accumulate = false) {
   const int m = int(gid / n_cells);
   const long c = gid - long(m) * n_cells;
   const int l = mom_l[m];
@@ -160,7 +160,7 @@ __device__ __forceinline__ void scatter_one(long gid, long n_cells, long n_dofs,
     double s = 0.0;
 #pragma unroll
     for (int j = 0; j < N; ++j) s += M[i * N + j] * xv[j];
-    sc[i] = T(s * sig);
+    sc[i] = accumulate ? T(double(sc[i]) + s * sig) : T(s * sig);
   }
 }
 
@@ -168,10 +168,10 @@ template <typename T, int N>
 __global__ __launch_bounds__(256) void k_scatter_source(
     long n_cells, int n_mom, long n_dofs, const double *__restrict__ x,
     const int *__restrict__ mat, const double *__restrict__ sigma_s, int L,
-    const int *__restrict__ mom_l, const double *Mm, T *__restrict__ S) {
+    const int *__restrict__ mom_l, const double *Mm, T *__restrict__ S, int accumulate) {
   const long gid = long(blockIdx.x) * blockDim.x + threadIdx.x;
   if (gid >= n_cells * n_mom) return;
-  scatter_one<T, N>(gid, n_cells, n_dofs, x, mat, sigma_s, L, mom_l, Mm, S);
+  scatter_one<T, N>(gid, n_cells, n_dofs, x, mat, sigma_s, L, mom_l, Mm, S, accumulate != 0);
 }
 
 // Moment 0 of the source sweep: S_0 + (M 1) q / (ws M2D[.,0]), so that
@@ -561,14 +561,15 @@ size_t lds_need(int dim, int p, int mm, int waves) {
 }
 
 template <typename T>
-void scatter_impl(Plan &pl, const double *d_x, hipStream_t s) {
+void scatter_impl(Plan &pl, const double *d_x, const double *d_sigma, bool accumulate,
+                  hipStream_t s) {
   const long n = pl.n_cells * pl.n_mom;
   T *S = static_cast<T *>(pl.d_S);
 #define OU_SCAT(NN)                                                                  \
   case NN:                                                                           \
     hipLaunchKernelGGL((k_scatter_source<T, NN>), dim3(grid_for(n, 256)), dim3(256), \
-                       0, s, pl.n_cells, pl.n_mom, pl.n_dofs, d_x, pl.d_mat,         \
-                       pl.d_sigma_s, pl.L, pl.d_mom_l, pl.d_mass, S);                \
+                       0, s, pl.n_cells, pl.n_mom, pl.n_dofs, d_x, pl.d_mat, d_sigma, \
+                       pl.L, pl.d_mom_l, pl.d_mass, S, int(accumulate));              \
     break;
   switch (pl.N) {
     OU_SCAT(4) OU_SCAT(9) OU_SCAT(16) OU_SCAT(8) OU_SCAT(27)
@@ -704,10 +705,15 @@ bool sweep_supported(int dim, int p, int mm, int precision, int waves, int max_l
 }
 
 void launch_scatter_source(Plan &pl, const double *d_x, hipStream_t s) {
+  launch_scatter_source_with(pl, d_x, pl.d_sigma_s, false, s);
+}
+
+void launch_scatter_source_with(Plan &pl, const double *d_x, const double *d_sigma,
+                                bool accumulate, hipStream_t s) {
   if (pl.precision == OU_FP32)
-    scatter_impl<float>(pl, d_x, s);
+    scatter_impl<float>(pl, d_x, d_sigma, accumulate, s);
   else
-    scatter_impl<double>(pl, d_x, s);
+    scatter_impl<double>(pl, d_x, d_sigma, accumulate, s);
   pl.S_is_zero = false;
 }
 

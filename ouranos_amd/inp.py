@@ -298,50 +298,67 @@ def resolve(name, parameters_path):
     return name
 
 
-def problem_from_parameters(path):
+def group_problems_from_parameters(path):
     """The problem a parameter file describes (ouranos.cc:255-330): its
     geometry and cross-section files, the LS or GLC quadrature of its Sn
-    order with L_max from the cross sections, one energy group.
+    order with L_max from the cross sections, one ProblemSpec per energy
+    group (that group's sigma_t, sigma_s(g->g), source and inflow), and the
+    group-to-group scattering tables: cross[g][gp][mat][l] = sigma_s(gp->g)
+    (RTMaterialProperties.hh:92-101 with (gp, g) as in
+    RadiativeTransfer.cc:869-870).
 
-    Returns (Parameters, ProblemSpec).  Deviations, each explicit: more than
-    one energy group, adaptive refinement cycles and reflective boundaries
-    raise OuError(UNSUPPORTED); a cell whose source id has no intensity in
-    the parameter file gets no source (the reference reads past its table)."""
+    Returns (Parameters, [ProblemSpec per group], cross).  Deviations, each
+    explicit: adaptive refinement cycles and reflective boundaries raise
+    OuError(UNSUPPORTED); a cell whose source id has no intensity in the
+    parameter file gets no source (the reference reads past its table)."""
     prm = Parameters(path)
-    dim = prm.get_dimension()
-    if prm.get_n_groups() != 1:
-        raise OuError(-4, "only one energy group is supported (multigroup outer iterations "
-                          "are not built)")
+    dim, ng = prm.get_dimension(), prm.get_n_groups()
     if prm.get_n_refinements() != 0:
         raise OuError(-4, "adaptive refinement cycles are not supported")
     geo = Geometry(resolve(prm.get_geometry_filename(), path), dim)
-    mp = RTMaterialProperties(resolve(prm.get_xs_filename(), path), geo.get_n_materials(),
-                              prm.get_n_groups())
+    mp = RTMaterialProperties(resolve(prm.get_xs_filename(), path), geo.get_n_materials(), ng)
     quad = Quadrature(prm.get_quad_type(), dim, sn=prm.get_sn_order(), L_max=mp.get_L_max(),
                       galerkin=prm.get_galerkin(), weight_sum=prm.get_weight_sum())
-    nm = geo.get_n_materials()
-    sigma_t = np.array([mp.get_sigma_t(m, 0) for m in range(nm)])
-    sigma_s = np.array([[mp.get_sigma_s(m, 0, 0, l) for l in range(mp.get_L_max() + 1)]
-                        for m in range(nm)])
-    intensity = np.array([prm.get_src(i, 0) for i in range(prm.get_n_src())])
-    sid = geo.cell_source
-    q = np.zeros(len(sid))
-    known = sid < len(intensity)
-    q[known] = intensity[sid[known]]
+    nm, nl = geo.get_n_materials(), mp.get_L_max() + 1
     bc = [BC_VACUUM] * 6
-    inc = [0.0] * 6
     for f in range(2 * dim):
         bc[f] = prm.get_bc_type(f)
         if bc[f] == BC_REFLECTIVE:
             raise OuError(-4, "reflective boundaries are not implemented (the reference "
                               "parses them but never applies them)")
-        if bc[f] in (BC_ISOTROPIC, BC_MOST_NORMAL):
-            inc[f] = prm.get_inc_flux(f, 0)
-    spec = ProblemSpec(os.path.basename(str(path)), dim, prm.get_fe_order(), geo.n_cells,
-                       geo.cell_size, quad, sigma_t, sigma_s, geo.cell_material.copy(),
-                       q, prm.get_weight_sum(), bc=bc, inc=inc)
-    return prm, spec
+    sid = geo.cell_source
+    specs = []
+    for g in range(ng):
+        sigma_t = np.array([mp.get_sigma_t(m, g) for m in range(nm)])
+        sigma_s = np.array([[mp.get_sigma_s(m, g, g, l) for l in range(nl)] for m in range(nm)])
+        intensity = np.array([prm.get_src(i, g) for i in range(prm.get_n_src())])
+        q = np.zeros(len(sid))
+        known = sid < len(intensity)
+        q[known] = intensity[sid[known]]
+        inc = [0.0] * 6
+        for f in range(2 * dim):
+            if bc[f] in (BC_ISOTROPIC, BC_MOST_NORMAL):
+                inc[f] = prm.get_inc_flux(f, g)
+        name = os.path.basename(str(path)) + (f"[g{g}]" if ng > 1 else "")
+        specs.append(ProblemSpec(name, dim, prm.get_fe_order(), geo.n_cells, geo.cell_size,
+                                 quad, sigma_t, sigma_s, geo.cell_material.copy(), q,
+                                 prm.get_weight_sum(), bc=list(bc), inc=inc))
+    cross = [[np.array([[mp.get_sigma_s(m, gp, g, l) for l in range(nl)] for m in range(nm)])
+              for gp in range(ng)] for g in range(ng)]
+    return prm, specs, cross
+
+
+def problem_from_parameters(path):
+    """group_problems_from_parameters for a one-group file: (Parameters,
+    ProblemSpec).  Several groups raise OuError(UNSUPPORTED) here; the driver
+    (ouranos_amd.ouranos) solves them with the outer group iteration."""
+    prm, specs, _ = group_problems_from_parameters(path)
+    if len(specs) != 1:
+        raise OuError(-4, "the parameter file has several energy groups: use "
+                          "group_problems_from_parameters")
+    return prm, specs[0]
 
 
 __all__ = ["Parameters", "Geometry", "RTMaterialProperties", "problem_from_parameters",
+           "group_problems_from_parameters",
            "resolve", "BC_REFLECTIVE", "BC_VACUUM", "BC_ISOTROPIC", "BC_MOST_NORMAL", "LS", "GLC"]

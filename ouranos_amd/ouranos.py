@@ -3,13 +3,20 @@
     python -m ouranos_amd.ouranos [--precision fp64|fp32] [--verbose] parameters.inp
 
 Reads the parameter file and the geometry and cross-section files it names
-(ouranos.cc:255-330), builds the plan and solves with the solver the file
-selects, as ouranos.cc:43-209 does for one energy group:
+(ouranos.cc:255-330), builds one plan per energy group and solves with the
+solver the file selects, as ouranos.cc:43-209 does:
 * SI: source iteration until ||phi_new - phi|| / ||phi_new|| < the inner
   tolerance, at most "Max inner iterations" sweeps (ouranos.cc:62-92);
-* GMRES / BICGSTAB: the Krylov solve of (I - D L^-1 M Sigma) phi = D L^-1 q
-  with "Max inner iterations" and the inner tolerance (ouranos.cc:131-171;
-  GMRES restarts at 30, deal.II's default).
+* GMRES / BICGSTAB: the Krylov solve of (I - D L^-1 M Sigma) phi = rhs with
+  "Max inner iterations" and the inner tolerance (ouranos.cc:131-171; GMRES
+  restarts at 30, deal.II's default).
+With several groups, the outer iteration (ouranos.cc:62-205) sweeps the
+groups in order; group g's right-hand side is D L^-1 (sum over the other
+groups g' of sigma_s(g'->g) M phi_g' + M q_g / ws + inflow_g), built from the
+latest fluxes (RadiativeTransfer.cc:845-876, with the stale-source defect
+A.11 fixed), and the iteration stops when sqrt(sum_g |phi_g - phi_g_old|^2 /
+sum_g |phi_g|^2) falls below the outer tolerance or after "Max outer
+iterations".  Each group's solve starts from its own latest flux.
 Writes the flux moments as the reference's output_results does (ouranos.cc:
 211-253): `<Output file>0000.vtk` with the point fields flux_000g_<mmm>m and
 the record `<Output file>.vtk`.  Also writes them, moment-major [m][dof] with
@@ -25,7 +32,7 @@ import time
 import numpy as np
 
 import ouranos_amd as ou
-from ouranos_amd.inp import Geometry, problem_from_parameters, resolve
+from ouranos_amd.inp import Geometry, group_problems_from_parameters, resolve
 
 SOLVER_NAMES = {ou.SI: "SI", ou.GMRES: "GMRES", ou.BICGSTAB: "BICGSTAB"}
 
@@ -48,28 +55,69 @@ def output_results(filename, spec, group_flux, origin=None):
     return str(filename) + "0000.vtk"
 
 
+def solve_groups(prm, specs, cross, precision=ou.FP64, verbose=False):
+    """The outer group iteration (ouranos.cc:62-205) on one plan per group.
+    Returns ([phi_g], summary)."""
+    ops = [s.operator(precision) for s in specs]
+    ng = len(ops)
+    phi = [np.zeros(op.n) for op in ops]
+    solver, tol, max_it = prm.get_solver_type(), prm.get_inner_tolerance(), prm.get_max_inner_it()
+    inner, outer_its, change, converged = [], 0, 0.0, True
+    for out_it in range(prm.get_max_outer_it() if ng > 1 else 1):
+        old = [p.copy() for p in phi]
+        for g, op in enumerate(ops):
+            if ng == 1:
+                phi[g], info = op.solve(solver=solver, tol=tol, max_it=max_it, restart=30,
+                                        phi0=phi[g], verbose=verbose, raise_on_fail=False)
+            else:
+                first = True
+                for gp in range(ng):
+                    if gp != g:
+                        op.scatter_source_with(phi[gp], cross[g][gp], accumulate=not first)
+                        first = False
+                rhs = op.sweep(None, with_source=True)
+                phi[g], info = op.solve_rhs(rhs, solver=solver, tol=tol, max_it=max_it,
+                                            restart=30, phi0=phi[g], verbose=verbose,
+                                            raise_on_fail=False)
+            inner.append(info)
+            converged &= info["converged"]
+        outer_its = out_it + 1
+        if ng > 1:
+            num = sum(float(np.sum((o - p) ** 2)) for o, p in zip(old, phi))
+            den = sum(float(np.sum(p ** 2)) for p in phi)
+            change = float(np.sqrt(num / den)) if den > 0 else 0.0
+            if verbose:
+                print(f"[ouranos] outer iteration {out_it}: change {change:.3e}", flush=True)
+            if change < prm.get_outer_tolerance():
+                break
+    if ng > 1:
+        converged = converged and change < prm.get_outer_tolerance()
+    summary = {"groups": ng, "outer_iterations": outer_its, "outer_change": change,
+               "inner_iterations": [i["iterations"] for i in inner],
+               "rel_residual": max(i["rel_residual"] for i in inner), "converged": converged}
+    return phi, summary
+
+
 def run(parameters_path, precision=ou.FP64, verbose=False, output=None):
-    """Solves the problem of one parameter file.  Returns (phi, summary)."""
-    prm, spec = problem_from_parameters(parameters_path)
+    """Solves the problem of one parameter file.  Returns (phi, summary):
+    phi is the moment vector of a one-group problem, else one per group."""
+    prm, specs, cross = group_problems_from_parameters(parameters_path)
+    spec = specs[0]
     geo = Geometry(resolve(prm.get_geometry_filename(), parameters_path), spec.dim)
-    op = spec.operator(precision)
     t0 = time.perf_counter()
-    phi, info = op.solve(solver=prm.get_solver_type(), tol=prm.get_inner_tolerance(),
-                         max_it=prm.get_max_inner_it(), restart=30, verbose=verbose,
-                         raise_on_fail=False)
+    phis, s = solve_groups(prm, specs, cross, precision, verbose)
     seconds = time.perf_counter() - t0
     summary = {"parameters": str(parameters_path),
                "solver": SOLVER_NAMES.get(prm.get_solver_type(), "?"),
                "dim": spec.dim, "fe_order": spec.p, "n_cells": list(spec.n),
                "n_dir": spec.quad.n_dir, "n_mom": spec.quad.n_mom, "unknowns": spec.size,
-               "iterations": info["iterations"], "rel_residual": info["rel_residual"],
-               "converged": info["converged"], "seconds": seconds}
+               "iterations": sum(s["inner_iterations"]), **s, "seconds": seconds}
     out = output if output is not None else prm.get_output_filename()
     if out:
-        np.save(out + ".npy", phi)
+        np.save(out + ".npy", np.stack(phis) if len(phis) > 1 else phis[0])
         summary["output"] = out + ".npy"
-        summary["vtk"] = output_results(out, spec, [phi], origin=geo.get_bottom_left())
-    return phi, summary
+        summary["vtk"] = output_results(out, spec, phis, origin=geo.get_bottom_left())
+    return (phis if len(phis) > 1 else phis[0]), summary
 
 
 def main(argv=None):

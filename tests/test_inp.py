@@ -277,3 +277,35 @@ def test_vtk_output(tmp_path, dim, p):
                 want = phi[m * spec.n_dofs + c * N + local]
                 assert v["fields"][f"flux_000g_{m:03d}m"][c * nv + k] == want
     assert open(str(tmp_path / "out.vtk")).read().split() == ["!NBLOCKS", "1", "out0000.vtk"]
+
+
+def multigroup_files(tmp_path, coupling=True, solver="GMRES"):
+    """The reference's integration-test problem with two energy groups:
+    sigma_t = (1, 1.5), sigma_s(g -> g') in RTMaterialProperties order
+    (mat, g, g', l), sources 1 and 0.5."""
+    ss = "0.5,0.3,0.1,0.6" if coupling else "0.5,0.,0.,0.6"
+    xs = _write(tmp_path, "xs2.inp", f"set L_max = 0\nset Sigma_t = 1.,1.5\nset Sigma_s = {ss}\n")
+    params = open(fixture("rt_parameters.inp")).read()
+    params = params.replace("./tests/rt_geometry.inp", fixture("rt_geometry.inp"))
+    params = params.replace("./tests/rt_xs.inp", xs)
+    params = params.replace("set Number of groups = 1", "set Number of groups = 2")
+    params = params.replace("set Source intensity = 1.", "set Source intensity = 1.,0.5")
+    params = params.replace("set Max outer iterations = 1", "set Max outer iterations = 100")
+    params = params.replace("set Tolerance outer solver = 1e-4", "set Tolerance outer solver = 1e-11")
+    params = params.replace("set Tolerance inner solver = 1e-9", "set Tolerance inner solver = 1e-12")
+    params = params.replace("set Max inner iterations = 100", "set Max inner iterations = 400")
+    params = params.replace("set Solver type = GMRES", f"set Solver type = {solver}")
+    return _write(tmp_path, "p2.inp", params)
+
+
+def test_group_problems_from_parameters(tmp_path):
+    prm, specs, cross = inp.group_problems_from_parameters(multigroup_files(tmp_path))
+    assert prm.get_n_groups() == 2 and len(specs) == 2
+    assert [s.sigma_t[0] for s in specs] == [1.0, 1.5]
+    assert [s.sigma_s[0, 0] for s in specs] == [0.5, 0.6]      # sigma_s(g -> g)
+    assert (specs[0].q == 1.0).all() and (specs[1].q == 0.5).all()
+    # cross[g][g'] = sigma_s(g' -> g): the file lists (g, g') = (0,1) as 0.3
+    assert cross[1][0][0, 0] == 0.3 and cross[0][1][0, 0] == 0.1
+    with pytest.raises(ou.OuError) as e:
+        inp.problem_from_parameters(multigroup_files(tmp_path))
+    assert e.value.code == -4
