@@ -83,6 +83,7 @@ def features_of(rk, mat):
 
 KRYLOV_STEPS = 90   # plain GMRES(30) applications per instance
 KRYLOV_STRIDE = 6   # record every 6th Arnoldi pair
+FGMRES_RESTART = 30  # restart length of the on-policy FGMRES pairs
 
 
 def pairs_of(op, spec, grid, seed, sources=("si", "krylov"), net=None):
@@ -90,8 +91,8 @@ def pairs_of(op, spec, grid, seed, sources=("si", "krylov"), net=None):
     Sources: "si" = the error-propagation chains of source iteration from the
     right-hand side and from a random field; "krylov" = the Arnoldi vectors
     of plain GMRES(30) on the instance's source problem; "fgmres" = the
-    preconditioned vectors of FGMRES(30) with `net` itself (the residuals the
-    network meets when it is used)."""
+    preconditioned vectors of FGMRES(FGMRES_RESTART) with `net` itself (the
+    residuals the network meets when it is used)."""
     _, mat = op.coarse_features(grid)
     chunks = []
     if "si" in sources:
@@ -105,7 +106,7 @@ def pairs_of(op, spec, grid, seed, sources=("si", "krylov"), net=None):
             raise ValueError("the fgmres source needs the network")
         chunks.append(op.learned_pairs_krylov(grid, KRYLOV_STEPS, stride=KRYLOV_STRIDE,
                                               max_records=KRYLOV_STEPS // KRYLOV_STRIDE,
-                                              net=net))
+                                              restart=FGMRES_RESTART, net=net))
     feats, targets = [], []
     for r, e in chunks:
         for rk, ek in zip(r, e):
@@ -266,7 +267,7 @@ def summarize(rows):
 
 
 def main(argv=None):
-    global KRYLOV_STEPS, KRYLOV_STRIDE
+    global KRYLOV_STEPS, KRYLOV_STRIDE, FGMRES_RESTART
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
     t = sub.add_parser("train")
@@ -287,6 +288,8 @@ def main(argv=None):
                    help="first training instance seed = TRAIN_SEED0 + offset")
     t.add_argument("--krylov-steps", type=int, default=KRYLOV_STEPS)
     t.add_argument("--krylov-stride", type=int, default=KRYLOV_STRIDE)
+    t.add_argument("--pair-restart", type=int, default=FGMRES_RESTART,
+                   help="restart length of the on-policy FGMRES pairs")
     t.add_argument("--eval-instances", type=int, default=0)
     e = sub.add_parser("evaluate")
     e.add_argument("--weights", required=True)
@@ -300,6 +303,7 @@ def main(argv=None):
 
     if a.cmd == "train":
         KRYLOV_STEPS, KRYLOV_STRIDE = a.krylov_steps, a.krylov_stride
+        FGMRES_RESTART = a.pair_restart
         fam = Family(a.family)
         sources = tuple(a.sources.split(","))
         net = ou.UNet(grid=a.grid, max_batch=a.batch, seed=a.seed)
@@ -332,8 +336,9 @@ def main(argv=None):
                                  f"{list(RECORD)})",
                            "krylov": f"plain GMRES(30) Arnoldi pairs (every {KRYLOV_STRIDE}th "
                                      f"of {KRYLOV_STEPS})",
-                           "fgmres": f"FGMRES(30) pairs (A z_j, z_j) with the network being "
-                                     f"trained (every {KRYLOV_STRIDE}th of {KRYLOV_STEPS})"},
+                           "fgmres": f"FGMRES({FGMRES_RESTART}) pairs (A z_j, z_j) with the "
+                                     f"network being trained (every {KRYLOV_STRIDE}th of "
+                                     f"{KRYLOV_STEPS})"},
                "sources_used": list(sources),
                "train_command": "python -m ouranos_amd.learned " + " ".join(argv or sys.argv[1:])}
         if a.eval_instances:
