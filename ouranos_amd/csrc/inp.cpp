@@ -338,6 +338,11 @@ Geometry::Geometry(const std::string &path, int dim_) : dim(dim_) {
     if (n_subdivisions[size_t(a)] < 1) fail(OU_ERR_ARG, "geometry: subdivisions must be >= 1");
     n_sub *= n_subdivisions[size_t(a)];
   }
+  // cells per axis after the refinements must fit an int
+  for (int a = 0; a < dim; ++a)
+    if (n_global_refinements > 30 ||
+        (long(n_subdivisions[size_t(a)]) << n_global_refinements) > (1L << 30))
+      fail(OU_ERR_ARG, "geometry: too many cells per axis after the global refinements");
   const std::vector<long> mids = list_integer(prm.get("Material IDs"), int(n_sub));
   const std::vector<long> sids = list_integer(prm.get("Source IDs"), int(n_sub));
   material_ids.assign(mids.begin(), mids.end());

@@ -309,3 +309,9 @@ def test_group_problems_from_parameters(tmp_path):
     with pytest.raises(ou.OuError) as e:
         inp.problem_from_parameters(multigroup_files(tmp_path))
     assert e.value.code == -4
+
+
+def test_refinement_overflow_is_an_error(tmp_path):
+    with pytest.raises(ou.OuError) as e:
+        inp.Geometry(_write(tmp_path, "g.inp", "set Number of global refinements = 40\n"), 2)
+    assert e.value.code == -1 and "too many cells" in str(e.value)
