@@ -160,3 +160,19 @@ def test_problem_builders_are_deterministic():
     assert h.quad.n_dir == 288 and h.quad.n_mom == 4 and h.sigma_t.shape[0] == 512
     assert h.updates_per_sweep() == 16307453952
     assert abs(h.algorithmic_bytes_per_sweep() / 165.7e9 - 1) < 0.01
+
+
+def test_missing_library_fails_loudly(tmp_path):
+    """No CPU fallback: without the built HIP library every entry point
+    raises instead of computing."""
+    saved_lib, saved_path = ou._lib, ou.LIB_PATH
+    try:
+        ou._lib = None
+        ou.LIB_PATH = str(tmp_path / "libouranos_amd.so")
+        with pytest.raises(ImportError):
+            ou.lib()
+        with pytest.raises(ImportError):
+            ou.Quadrature(ou.GLC, 2, sn=4)
+    finally:
+        ou._lib, ou.LIB_PATH = saved_lib, saved_path
+    assert ou.lib() is not None
