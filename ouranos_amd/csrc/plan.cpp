@@ -331,6 +331,24 @@ Plan *create_plan(const ou_problem_desc *desc) {
     p.level_chunk_off[p.n_levels] = int(chunks.size());
     p.n_chunks = long(chunks.size());
 
+    // ---- direction groups ----------------------------------------------------
+    // Enough workgroups per level launch to give each CU about two (a 2D level
+    // or a small mesh has few chunks), but at least one full round of
+    // directions per workgroup: each group re-reads its chunk's source and
+    // writes its own partial moments.
+    {
+      int dev = 0, cus = 256;
+      if (hipGetDevice(&dev) == hipSuccess)
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+      int max_dirs = 1;
+      for (int o = 0; o < p.n_oct; ++o) max_dirs = std::max(max_dirs, p.oct_off[o + 1] - p.oct_off[o]);
+      const double per_level = double(std::max<long>(p.n_chunks, 1)) / std::max(p.n_levels, 1);
+      const int g_fill = int(std::ceil(2.0 * std::max(cus, 1) / per_level));
+      const int g_dirs = std::max(1, max_dirs / std::max(p.waves, 1));
+      p.dir_groups = std::max(1, std::min({g_fill, g_dirs, kMaxDirGroups}));
+      p.partial_groups = p.dir_groups;
+    }
+
     // ---- boundary inflow per (direction, axis) ----------------------------
     std::vector<double> inflow(3 * size_t(p.n_dir), 0.0);
     for (int d = 0; d < p.n_dir; ++d)
@@ -394,8 +412,9 @@ Plan *create_plan(const ou_problem_desc *desc) {
     p.d_slot_c = upload_real(slot_c, p.precision);
     p.d_slot_d2m = static_cast<double *>(dev_alloc(slot_d2m.size() * 8));
     upload(p.d_slot_d2m, slot_d2m);
-    p.d_partial = static_cast<double *>(dev_alloc(size_t(p.n_oct) * p.vec_size * 8));
-    OU_HIP(hipMemset(p.d_partial, 0, size_t(p.n_oct) * p.vec_size * 8));
+    p.d_partial = static_cast<double *>(
+        dev_alloc(size_t(p.n_oct) * p.partial_groups * p.vec_size * 8));
+    OU_HIP(hipMemset(p.d_partial, 0, size_t(p.n_oct) * p.partial_groups * p.vec_size * 8));
     p.plane_size[0] = long(ny) * nz;
     p.plane_size[1] = long(nx) * nz;
     p.plane_size[2] = long(nx + ny - 1) * ny;

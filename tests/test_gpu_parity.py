@@ -226,3 +226,20 @@ def test_apply_properties_at_full_2d_size():
     spec.sigma_s = np.zeros_like(spec.sigma_s)
     op0 = spec.operator(ou.FP32)
     assert np.array_equal(op0.Apply(x), x)
+
+
+def test_default_direction_groups_follow_level_parallelism(oracle):
+    """A 2D mesh has few chunks per wavefront level: the default shape splits
+    each quadrant's directions over several workgroups.  Results differ from
+    one group only in the order of the FP64 partial sums, and match the oracle."""
+    spec = P.hetero_2d(n=64, block=8, n_polar=8, n_azim=16, L=1)
+    op = spec.operator(ou.FP64)
+    w, g = op.sweep_shape
+    assert g > 1, (w, g)
+    x = P.seeded_vector(spec.size, 5)
+    y = op.Apply(x)
+    assert rel_err(y, oracle_problem(oracle, spec).apply(x)) < FP64_TOL
+    op.sweep_shape = (w, 1)
+    assert rel_err(op.Apply(x), y) < 1e-14
+    # the headline's level launches already fill the GPU: one group
+    assert P.headline_3d(n=5, block=2, n_polar=4, n_azim=8, L=1).operator(ou.FP32).sweep_shape[1] == 1
