@@ -228,6 +228,32 @@ def test_apply_properties_at_full_2d_size():
     assert np.array_equal(op0.Apply(x), x)
 
 
+def test_apply_properties_at_full_headline_size():
+    """128^3 Q2, GLC 12x24 (288 directions), L=1, the size no CPU
+    restatement finishes in seconds: FP64 linearity, the FP32 sweep part
+    within the FP32 bound of the FP64 one, and sigma_s = 0 => identity."""
+    spec = P.headline_3d()
+    print(f"[gpu] full-size 3D properties: {spec.size} unknowns", flush=True)
+    x = P.seeded_vector(spec.size, 1)
+    y = P.seeded_vector(spec.size, 2)
+    op = spec.operator(ou.FP64)
+    ax, ay = op.Apply(x), op.Apply(y)
+    assert np.isfinite(ax).all()
+    z = 2.0 * x - 0.5 * y
+    err = rel_err(z - op.Apply(z), z - (2.0 * ax - 0.5 * ay))
+    print(f"[gpu] full-size FP64 linearity (sweep part): {err:.2e}", flush=True)
+    assert err < 1e-12
+    del op, y, ay, z
+    op32 = spec.operator(ou.FP32)
+    err = rel_err(x - op32.Apply(x), x - ax)
+    print(f"[gpu] full-size FP32 sweep part vs FP64: {err:.2e}", flush=True)
+    assert err < FP32_TOL
+    del op32, ax
+    spec.sigma_s = np.zeros_like(spec.sigma_s)
+    op0 = spec.operator(ou.FP32)
+    assert np.array_equal(op0.Apply(x), x)
+
+
 def test_default_direction_groups_follow_level_parallelism(oracle):
     """A 2D mesh has few chunks per wavefront level: the default shape splits
     each quadrant's directions over several workgroups.  Results differ from
