@@ -310,7 +310,10 @@ int ou_solve_refined(ou_plan *plan64, ou_plan *plan32, ou_unet *net,
 /* Waves per sweep workgroup (directions in flight per chunk of 64 cells) and
  * direction groups per octant (workgroups per chunk, each with a range of the
  * octant's directions and its own FP64 partial moments, summed in the
- * epilogue).  Shapes change the order of FP64 additions only. */
+ * epilogue).  Shapes change the order of FP64 additions only.  dir_groups
+ * lies in [1, 8] (OU_ERR_ARG otherwise); waves must name a kernel built for
+ * the element (OU_ERR_UNSUPPORTED otherwise).  A new plan starts with the
+ * groups its level parallelism calls for. */
 int ou_set_sweep_shape(ou_plan *plan, int waves, int dir_groups);
 int ou_sweep_shape(const ou_plan *plan, int *waves, int *dir_groups);
 
