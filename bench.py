@@ -8,7 +8,19 @@ mesh, DG Q2 (27 dofs/cell), Gauss-Legendre-Chebyshev 12x24 = 288 directions,
 L=1 anisotropic scattering, seeded 16^3-cell material blocks; FP32 angular
 flux with FP64 moment accumulation.
 
-Prints ONE JSON line.  `--gpus N` with N > 1 exits: this build runs on one GPU.
+The same line carries the rest of the metric, measured in the same run:
+  time_to_tol   time to a 1e-8 relative residual (FP64 residual of the FP64
+                operator) on the same workload: plain FP64 GMRES(30) against
+                the learned right preconditioner in FGMRES(16) on the FP32
+                sweep with FP64 refinement; seconds, iterations, residuals,
+                scalar-flux difference and the ratio
+  fp64_sweep    the FP64 operator application (the reference's precision):
+                ms, DoF-angle updates/s and its HBM roofline by FP64 bytes
+  cpu_baseline  the FP64 oracle sweep on the host's cores: one thread and the
+                box's CPU share, with the core counts seen
+
+Prints ONE JSON line on stdout (progress goes to stderr).  `--gpus N` with
+N > 1 exits: this build runs on one GPU.
 """
 import argparse
 import json
@@ -24,6 +36,13 @@ sys.path.insert(0, ROOT)
 
 METRIC = "sweep DoF-angle updates/s; time-to-1e-8 learned-FGMRES vs plain GMRES"
 HBM_PEAK_GBS = 8000.0
+# host threads that one GPU's share of the box may use
+CPU_SHARE = 16
+WEIGHTS = os.path.join(ROOT, "ouranos_amd", "weights", "unet_c5_s4.npy")
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
 
 
 def workload(name):
@@ -61,15 +80,14 @@ def cpu_baseline(spec, seconds_budget, threads):
     """Times the FP64 oracle sweep (plain C restatement of
     RadiativeTransfer::sweep, one cell per task, Crout LU per cell) on a slab
     of the same mesh: the first z-layers (all of x, y) with the same
-    materials, one direction per thread.  Returns updates/s over all threads."""
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    materials, one direction per thread, each direction sized to take about
+    `seconds_budget`.  Returns updates/s over all threads."""
     from oracle import oracle as O
     O.lib()
     n = list(spec.n)
-    # slab size chosen so one direction takes roughly seconds_budget/2
     per_cell_s = 5e-6 * (spec.n_loc / 27.0) ** 2.2  # measured 5.2 us per Q2 cell-direction
-    layers = max(1, min(n[-1], int(seconds_budget / 2 / per_cell_s / (n[0] * n[1]))))
-    sub_n = n[:-1] + [layers] if spec.dim == 3 else n[:-1] + [layers]
+    layers = max(1, min(n[-1], int(seconds_budget / per_cell_s / (n[0] * n[1]))))
+    sub_n = n[:-1] + [layers]
     nc = int(np.prod(sub_n))
     q = spec.quad
     pb = O.Problem(spec.dim, spec.p, sub_n, spec.h, q.omega, q.M2D, q.D2M, q.mom_l,
@@ -96,6 +114,99 @@ def cpu_baseline(spec, seconds_budget, threads):
                        f"{updates:.3e} updates in {dt:.2f} s")}
 
 
+def cpu_legs(spec, seconds_budget):
+    """The oracle timed on one thread and on the host threads of one GPU's
+    share of the box (at most CPU_SHARE, and no more than the process may run
+    on).  The multi-thread leg is the reported baseline; both legs and the
+    core counts the host shows are kept."""
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = os.cpu_count() or 1
+    threads = max(1, min(CPU_SHARE, affinity))
+    one = cpu_baseline(spec, seconds_budget / 2, 1)
+    many = cpu_baseline(spec, seconds_budget / 2, threads)
+    out = dict(many)
+    out["single_thread"] = {"value": one["value"], "unit": one["unit"], "cores": 1,
+                            "sample": one["sample"]}
+    out["parallel_speedup"] = many["value"] / one["value"]
+    out["host_cpus"] = {"os_cpu_count": os.cpu_count(), "affinity": affinity,
+                        "used": threads, "share_per_gpu": CPU_SHARE}
+    return out
+
+
+def time_to_tolerance(spec, op32, tol):
+    """Time to a `tol` relative residual, every residual test FP64 on the FP64
+    operator: plain FP64 GMRES(30), the reference's Krylov method with the
+    identity preconditioner (ouranos.cc:131-208), against FGMRES(16) on the
+    FP32 operator with the learned right preconditioner and FP64 refinement."""
+    import ouranos_amd as ou
+    from ouranos_amd import learned as Lm
+    op64 = spec.operator(ou.FP64)
+    nd = spec.n_dofs
+    log("[bench] time to tolerance: plain FP64 GMRES(30)")
+    t0 = time.perf_counter()
+    phi_p, info_p = op64.solve(solver=ou.GMRES, tol=tol, max_it=3000, restart=30,
+                               raise_on_fail=False)
+    wall_p = time.perf_counter() - t0
+    ref = phi_p[:nd].copy()
+    del phi_p
+    log(f"[bench]   {info_p['iterations']} iterations, {info_p['seconds']:.2f} s")
+    net = Lm.load(WEIGHTS)
+    log("[bench] time to tolerance: learned FGMRES(16), FP32 sweep, FP64 refinement")
+    t0 = time.perf_counter()
+    phi_l, info_l = ou.solve_refined(op64, op32, net, tol=tol, max_it=3000, restart=16,
+                                     inner_tol=3e-5, raise_on_fail=False)
+    wall_l = time.perf_counter() - t0
+    diff = float(np.linalg.norm(phi_l[:nd] - ref) / np.linalg.norm(ref))
+    del phi_l
+    log(f"[bench]   {info_l['iterations']} iterations, {info_l['seconds']:.2f} s")
+    op64.close()
+    return {
+        "tol": tol,
+        "plain": {"method": "FP64 GMRES(30), FP64 sweep", "seconds": info_p["seconds"],
+                  "wall_seconds": wall_p, "iterations": info_p["iterations"],
+                  "fp64_rel_residual": info_p["rel_residual"],
+                  "converged": bool(info_p["converged"])},
+        "learned": {"method": ("FGMRES(16) on the FP32 sweep (FP64 moments), learned right "
+                               "preconditioner (U-Net, BF16), FP64 refinement"),
+                    "weights": os.path.relpath(WEIGHTS, ROOT),
+                    "seconds": info_l["seconds"], "wall_seconds": wall_l,
+                    "iterations": info_l["iterations"],
+                    "fp64_applications": info_l.get("fp64_applications"),
+                    "fp64_rel_residual": info_l["rel_residual"],
+                    "converged": bool(info_l["converged"]),
+                    "scalar_flux_rel_diff_vs_plain": diff},
+        "ratio": info_p["seconds"] / info_l["seconds"],
+    }
+
+
+def fp64_leg(spec, d_x, d_y, warmup, steps):
+    """The operator application in FP64 throughout (the reference's
+    precision) on the same vectors: ms per application, updates/s, and the
+    sweep kernels' HBM roofline by the FP64 form of the traffic formula."""
+    import ouranos_amd as ou
+    log("[bench] FP64 sweep leg")
+    op64 = spec.operator(ou.FP64)
+    if warmup > 0:
+        op64.time_apply(d_x, d_y, warmup, 1)
+    ou.lib().ou_device_synchronize()
+    t0 = time.perf_counter()
+    tm = op64.time_apply(d_x, d_y, 0, steps)
+    ou.lib().ou_device_synchronize()
+    wall = time.perf_counter() - t0
+    op64.close()
+    sweep_ms = tm["ms_sweep_kernel"] / steps
+    alg = spec.algorithmic_bytes_per_sweep(real_bytes=8)
+    achieved = alg / (sweep_ms / 1000.0) / 1e9
+    return {"dtype": "fp64", "steps": steps, "ms_per_step": 1000.0 * wall / steps,
+            "value": spec.updates_per_sweep() * steps / wall, "unit": "DoF-angle updates/s",
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                         "algorithmic_bytes_per_sweep": alg,
+                         "sweep_kernel_ms_per_step": sweep_ms}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -104,8 +215,11 @@ def main():
     ap.add_argument("--workload", default="headline_3d")
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
-    ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-solve", action="store_true",
+                    help="skip the time-to-tolerance comparison")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 sweep leg")
+    ap.add_argument("--tol", type=float, default=1e-8)
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path "
@@ -117,6 +231,7 @@ def main():
 
     spec, desc = workload(a.workload)
     prec = ou.FP32 if a.precision == "fp32" else ou.FP64
+    log(f"[bench] {spec.name}: building the plan")
     t0 = time.perf_counter()
     op = spec.operator(prec)
     t_plan = time.perf_counter() - t0
@@ -126,6 +241,7 @@ def main():
     d_x.upload(x)
     del x
     # warmup + timed region, device synchronised on both sides
+    log(f"[bench] {a.warmup} warmup + {a.steps} timed applications")
     op.time_apply(d_x, d_y, a.warmup, 1) if a.warmup > 0 else None
     ou.lib().ou_device_synchronize()
     t0 = time.perf_counter()
@@ -136,12 +252,13 @@ def main():
     d_y.download(y)
     if not np.isfinite(y).all():
         raise SystemExit("non-finite output")
+    del y
 
     updates = spec.updates_per_sweep()
     ms_step = 1000.0 * wall / a.steps
     value = updates * a.steps / wall
     sweep_ms = tm["ms_sweep_kernel"] / a.steps
-    alg_bytes = spec.algorithmic_bytes_per_sweep()
+    alg_bytes = spec.algorithmic_bytes_per_sweep(real_bytes=4 if prec == ou.FP32 else 8)
     achieved = alg_bytes / (sweep_ms / 1000.0) / 1e9
     traffic, traffic_src = measured_traffic(spec.name, a.precision)
     line = {
@@ -156,17 +273,27 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "traffic_source": traffic_src,
-                     "kernel": "k_sweep_level (all level launches of one sweep)",
+                     "kernel": tm.get("sweep_kernel", "k_sweep_level (one sweep)"),
                      "algorithmic_bytes_per_sweep": alg_bytes,
                      "sweep_kernel_ms_per_step": sweep_ms,
                      "launches_per_step": tm["n_sweep_launches"]},
         "event_ms_per_step": tm["ms_total"] / a.steps,
         "plan_seconds": t_plan,
     }
+    if not a.no_fp64 and prec == ou.FP32:
+        line["fp64_sweep"] = fp64_leg(spec, d_x, d_y, a.warmup, a.steps)
+    d_x.free()
+    d_y.free()
+    if not a.no_solve and spec.name == "headline_3d":
+        try:
+            line["time_to_tol"] = time_to_tolerance(spec, op, a.tol)
+        except Exception as e:  # reported, not hidden: the line still prints
+            line["time_to_tol"] = {"error": f"{type(e).__name__}: {e}"}
+    op.close()
     if not a.no_cpu:
-        threads = a.cpu_threads or min(16, os.cpu_count() or 1)
-        line["cpu_baseline"] = cpu_baseline(spec, a.cpu_seconds, threads)
-    print(json.dumps(line))
+        log("[bench] CPU baseline")
+        line["cpu_baseline"] = cpu_legs(spec, a.cpu_seconds)
+    print(json.dumps(line), flush=True)
 
 
 if __name__ == "__main__":

@@ -63,6 +63,17 @@ def gauss_legendre(n):
     return x, w
 
 
+def ls_octant(sn):
+    """One octant of the level-symmetric set as LS::build_octant leaves it:
+    (omega [n, 3], raw weights [n]) before deployment and normalisation."""
+    om = np.zeros(3 * 64)
+    w = np.zeros(64)
+    n = lib().orc_ls_octant(sn, _d(om), _d(w))
+    if n < 0:
+        raise ValueError("unsupported quadrature order")
+    return om[: 3 * n].reshape(n, 3).copy(), w[:n].copy()
+
+
 def quadrature(kind, dim, sn=None, n_polar=None, n_azim=None):
     """Deployed direction set: kind 'ls' | 'glc' (triangular) | 'product'."""
     cap = 8 * 4096

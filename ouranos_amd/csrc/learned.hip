@@ -20,6 +20,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <string>
 
 #include "learned.hpp"
 #include "vec.hpp"
@@ -296,6 +297,7 @@ RefinedResult solve_refined(RadiativeTransfer &rt64, RadiativeTransfer &rt32,
   RefinedResult res;
   build_rhs(rt64, b, s);
   const double bnorm = norm(b);
+  if (!std::isfinite(bnorm)) fail(OU_ERR_NONFINITE, "non-finite right-hand side norm");
   if (bnorm == 0.0) {
     vec_fill(n, 0.0, d_phi, s);
     res.total.converged = true;
@@ -316,6 +318,9 @@ RefinedResult solve_refined(RadiativeTransfer &rt64, RadiativeTransfer &rt32,
       ++res.outer;
     }
     const double rn = norm(r);
+    if (!std::isfinite(rn))
+      fail(OU_ERR_NONFINITE, "non-finite FP64 residual norm in refinement cycle " +
+                                 std::to_string(cycle));
     fp64_secs += std::chrono::duration<double>(std::chrono::steady_clock::now() - a0).count();
     res.total.rel_residual = rn / bnorm;
     if (verbose)

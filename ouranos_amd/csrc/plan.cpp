@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <cstdint>
 #include <cstdlib>
 #include <cstring>
 #include <numeric>
@@ -100,6 +101,22 @@ Plan *create_plan(const ou_problem_desc *desc) {
     fail(OU_ERR_ARG, "material arrays missing");
   if (ds.L < 0) fail(OU_ERR_ARG, "L must be >= 0");
   if (!(ds.weight_sum > 0.0)) fail(OU_ERR_ARG, "weight_sum must be positive");
+  // The schedule and the kernels index cells and trace slots in 32 bits: the
+  // cell list holds every (octant, cell) as an int, a kernel's cell id is an
+  // int, and a trace value sits at unsigned (dir F + f) plane_size + slot.
+  // Larger meshes are refused here, before any array is read or allocated.
+  {
+    const long nx = ds.n_cells[0], ny = ds.n_cells[1], nz = ds.dim == 3 ? ds.n_cells[2] : 1;
+    const long n_oct = ds.dim == 2 ? 4 : 8;
+    if (nx * ny * nz * n_oct > long(INT32_MAX))
+      fail(OU_ERR_UNSUPPORTED, "mesh too large: octants x cells must fit a 32-bit index");
+    const long P1 = ds.fe_order + 1, F = ds.dim == 2 ? P1 : P1 * P1;
+    const long planes[3] = {ny * nz, nx * nz, (nx + ny - 1) * ny};
+    for (int a = 0; a < ds.dim; ++a)
+      if (long(ds.n_dir) * F * planes[a] > long(UINT32_MAX))
+        fail(OU_ERR_UNSUPPORTED, "mesh too large: a direction set of face traces must fit "
+                                 "a 32-bit index");
+  }
 
   auto *pl = new Plan();
   try {
@@ -469,6 +486,14 @@ void set_sweep_shape(Plan &pl, int waves, int dir_groups) {
     (void)hipFree(pl.d_partial);
     pl.d_partial = static_cast<double *>(fresh);
     pl.partial_groups = dir_groups;
+  }
+  if (dir_groups != pl.dir_groups) {
+    // The epilogue sums the slots of all octants x groups; an octant without
+    // directions has no chunks, so nothing writes its slots.  Its slots must
+    // hold zeros in the new (octant, group) layout, whose offsets differ from
+    // the old one's.
+    OU_HIP(hipDeviceSynchronize());
+    OU_HIP(hipMemset(pl.d_partial, 0, size_t(pl.n_oct) * pl.partial_groups * pl.vec_size * 8));
   }
   pl.waves = waves;
   pl.dir_groups = dir_groups;

@@ -56,6 +56,17 @@ struct Dotter {
   double norm(long n, const double *a, hipStream_t s) { return std::sqrt(dot(n, a, a, s)); }
 };
 
+// A value the device produced that is not finite ends the solve at once: no
+// later iteration can recover from it, and the residual tests (x < tol) are
+// false for NaN, so the loop would otherwise run to its iteration limit.
+void check_finite(double v, const char *what, int it) {
+  if (!std::isfinite(v)) {
+    char buf[160];
+    std::snprintf(buf, sizeof buf, "non-finite %s (%g) at iteration %d of the solve", what, v, it);
+    fail(OU_ERR_NONFINITE, buf);
+  }
+}
+
 }  // namespace
 
 void build_rhs(RadiativeTransfer &rt, double *d_rhs, hipStream_t s) {
@@ -79,6 +90,7 @@ SolveResult solve_si(RadiativeTransfer &rt, double *d_phi, double tol, int max_i
     vec_axpby(n, 1.0, nw.p, -1.0, d_phi, s);  // d_phi <- new - old
     const double num = dt.norm(n, d_phi, s);
     const double den = dt.norm(n, nw.p, s);
+    check_finite(num + den, "change norm", it);
     vec_copy(n, nw.p, d_phi, s);
     res.iterations = it + 1;
     res.rel_residual = den > 0.0 ? num / den : 0.0;
@@ -105,6 +117,7 @@ SolveResult solve_si_rhs(Operator &op, const double *d_b, double *d_x, double to
     vec_axpby(n, 1.0, nw.p, -1.0, d_x, s);          // d_x <- new - old
     const double num = dt.norm(n, d_x, s);
     const double den = dt.norm(n, nw.p, s);
+    check_finite(num + den, "change norm", it);
     vec_copy(n, nw.p, d_x, s);
     res.iterations = it + 1;
     res.rel_residual = den > 0.0 ? num / den : 0.0;
@@ -132,6 +145,7 @@ SolveResult solve_gmres(Operator &op, const double *d_b, double *d_x, int restar
   std::vector<double> H(size_t(m + 1) * m, 0.0), cs(m), sn(m), g(m + 1), y(m), h(m + 1);
   SolveResult res;
   const double bnorm = dt.norm(n, d_b, s);
+  check_finite(bnorm, "right-hand side norm", 0);
   if (bnorm == 0.0) {
     vec_fill(n, 0.0, d_x, s);
     res.converged = true;
@@ -151,6 +165,7 @@ SolveResult solve_gmres(Operator &op, const double *d_b, double *d_x, int restar
       vec_lincomb(n, 1.0, d_b, -1.0, V.p, V.p, s);
     }
     const double beta = dt.norm(n, V.p, s);
+    check_finite(beta, "residual norm", total);
     res.rel_residual = beta / bnorm;
     if (verbose)
       progress("GMRES restart at %d: true residual %.3e (%d reorthogonalised steps)\n", total,
@@ -199,6 +214,11 @@ SolveResult solve_gmres(Operator &op, const double *d_b, double *d_x, int restar
           hn = std::sqrt(std::max(h3[0], 0.0));
           ++reorth;
         }
+      }
+      {
+        double hsum = hn;
+        for (int i = 0; i <= j; ++i) hsum += std::fabs(h[i]);
+        check_finite(hsum, "Arnoldi coefficient", total);
       }
       for (int i = 0; i <= j; ++i) H[size_t(i) * m + j] = h[i];
       H[size_t(j + 1) * m + j] = hn;
@@ -259,11 +279,14 @@ SolveResult solve_bicgstab(Operator &op, const double *d_b, double *d_x, double 
   vec_fill(n, 0.0, p.p, s);
   vec_fill(n, 0.0, v.p, s);
   double rho = 1.0, alpha = 1.0, omega = 1.0;
+  check_finite(bnorm, "right-hand side norm", 0);
   res.rel_residual = dt.norm(n, r.p, s) / bnorm;
+  check_finite(res.rel_residual, "residual norm", 0);
   int it = 0;
   while (res.rel_residual >= tol && it < max_it) {
     ++it;
     const double rho1 = dt.dot(n, r0.p, r.p, s);
+    check_finite(rho1, "rho", it);
     const double beta = (rho1 / rho) * (alpha / omega);
     rho = rho1;
     // p = r + beta (p - omega v)
@@ -273,6 +296,7 @@ SolveResult solve_bicgstab(Operator &op, const double *d_b, double *d_x, double 
     alpha = rho / dt.dot(n, r0.p, v.p, s);
     vec_lincomb(n, 1.0, r.p, -alpha, v.p, sv.p, s);
     const double snorm = dt.norm(n, sv.p, s);
+    check_finite(alpha + snorm, "alpha or |s|", it);
     if (snorm / bnorm < tol) {
       vec_axpby(n, alpha, p.p, 1.0, d_x, s);
       res.rel_residual = snorm / bnorm;
@@ -281,10 +305,12 @@ SolveResult solve_bicgstab(Operator &op, const double *d_b, double *d_x, double 
     op.Apply(sv.p, t.p, s);
     const double ts = dt.dot(n, t.p, sv.p, s), tt = dt.dot(n, t.p, t.p, s);
     omega = ts / tt;
+    check_finite(omega, "omega", it);
     vec_axpby(n, alpha, p.p, 1.0, d_x, s);
     vec_axpby(n, omega, sv.p, 1.0, d_x, s);
     vec_lincomb(n, 1.0, sv.p, -omega, t.p, r.p, s);
     res.rel_residual = dt.norm(n, r.p, s) / bnorm;
+    check_finite(res.rel_residual, "residual norm", it);
     if (verbose) progress("BiCGStab it %d: %.3e\n", it, res.rel_residual);
   }
   res.iterations = it;

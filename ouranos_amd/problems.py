@@ -90,11 +90,13 @@ class ProblemSpec:
     def updates_per_sweep(self):
         return self.n_dofs * self.quad.n_dir
 
-    def algorithmic_bytes_per_sweep(self):
+    def algorithmic_bytes_per_sweep(self, real_bytes=4):
         """HBM bytes of one sweep by the fixed traffic formula: every interior
-        face trace written and read once in FP32 (8 B per value), plus per
-        octant batch an FP32 read of the moment source and an FP64
-        read-modify-write of the moment accumulators (20 B per value)."""
+        face trace written and read once in the sweep precision (2 x 4 B per
+        value in FP32), plus per octant batch a read of the moment source in
+        the sweep precision and an FP64 read-modify-write of the moment
+        accumulators (4 + 16 B per value in FP32).  real_bytes = 8 gives the
+        same formula for the FP64 sweep."""
         n = list(self.n)
         f_int = 0
         for a in range(self.dim):
@@ -104,8 +106,8 @@ class ProblemSpec:
             f_int += prod
         f_v = (self.p + 1) ** (self.dim - 1)
         n_batch = 4 if self.dim == 2 else 8
-        return (self.quad.n_dir * f_int * f_v * 8
-                + n_batch * self.n_cells * self.n_loc * self.quad.n_mom * 20)
+        return (self.quad.n_dir * f_int * f_v * 2 * real_bytes
+                + n_batch * self.n_cells * self.n_loc * self.quad.n_mom * (real_bytes + 16))
 
 
 def _block_map(n, block):

@@ -7,6 +7,7 @@ and writes small fixtures next to this script:
   fecell_q1.json              Q1 matrices/normals, tests/FECell_test.cc:52-297
   quadrature_s4.json          LS/GLC S4 checks, tests/RTQuadrature_test.cc:21-165
   gauss_legendre_tables.json  polar nodes/weights of GLC.cc:60-175 (orders 2..16)
+  ls_tables.json              level-symmetric octant tables of LS.cc:14-169 (orders 2..12)
 
 Usage: python tests/golden/make_fixtures.py [/root/reference]
 """
@@ -135,6 +136,35 @@ def gl_tables(ref):
     return out
 
 
+def ls_tables(ref):
+    """The level-symmetric octant data of LS::build_octant (LS.cc:14-169),
+    orders 2..12, as printed: the distinct direction cosines direction[k],
+    the weight classes w_k, and weight(i) of every octant direction i in the
+    order LS::compute_omega lays them out."""
+    src = open(os.path.join(ref, "src/LS.cc")).read()
+    body = src.split("void LS::build_octant()")[1].split("void LS::compute_omega")[0]
+    out = {"source": "src/LS.cc:14-169"}
+    for case in re.finditer(r"case (\d+)\s*:\s*\{(.*?)break;", body, re.S):
+        sn = int(case.group(1))
+        if sn > 12:
+            continue
+        text = case.group(2)
+        if sn == 2:
+            d = re.search(r"direction\(([0-9.]+)\)", text).group(1)
+            w = re.search(r"const double w\(([0-9./]+)\)", text).group(1)
+            out["2"] = {"cosines": [float(d)], "classes": {"w": float(eval(w))},
+                        "weights": ["w"]}
+            continue
+        cos = {int(i): float(v) for i, v in re.findall(r"direction\[(\d+)\]\s*=\s*([0-9.]+)", text)}
+        classes = {}
+        for name, v in re.findall(r"const double (w(?:_\d+)?)\(([0-9./]+)\)", text):
+            classes[name] = float(eval(v, {"__builtins__": {}}))  # decimal or a/b literals only
+        wts = {int(i): name for i, name in re.findall(r"weight\((\d+)\)\s*=\s*(w(?:_\d+)?);", text)}
+        out[str(sn)] = {"cosines": [cos[i] for i in sorted(cos)], "classes": classes,
+                        "weights": [wts[i] for i in sorted(wts)]}
+    return out
+
+
 def main():
     ref = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
     fixtures = {
@@ -142,6 +172,7 @@ def main():
         "fecell_q1.json": fecell(ref),
         "quadrature_s4.json": quadrature(ref),
         "gauss_legendre_tables.json": gl_tables(ref),
+        "ls_tables.json": ls_tables(ref),
     }
     for name, data in fixtures.items():
         with open(os.path.join(HERE, name), "w") as f:

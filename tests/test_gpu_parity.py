@@ -269,3 +269,61 @@ def test_default_direction_groups_follow_level_parallelism(oracle):
     assert rel_err(op.Apply(x), y) < 1e-14
     # the headline's level launches already fill the GPU: one group
     assert P.headline_3d(n=5, block=2, n_polar=4, n_azim=8, L=1).operator(ou.FP32).sweep_shape[1] == 1
+
+
+class _Subset:
+    """A direction set made of some directions of a quadrature (the operator
+    takes any omega / M2D / D2M it is handed)."""
+
+    def __init__(self, q, keep):
+        self.omega = np.ascontiguousarray(q.omega[keep])
+        self.M2D = np.ascontiguousarray(q.M2D[keep])
+        self.D2M = np.ascontiguousarray(q.D2M[:, keep])
+        self.mom_l = q.mom_l
+        self.most_normal = np.ascontiguousarray(q.most_normal[:, keep])
+        self.weight_sum = q.weight_sum
+        self.n_dir = int(self.omega.shape[0])
+        self.n_mom = q.n_mom
+
+
+def test_reshape_with_an_empty_octant_matches_oracle(oracle):
+    """A direction set with no direction in one octant: that octant has no
+    chunks, so nothing writes its partial-moment slots.  Changing the
+    direction groups (1 -> 4 -> 1 -> 2) changes the slot layout; every shape
+    still matches the oracle."""
+    q = ou.Quadrature(ou.LS, 3, sn=6, L_max=1)
+    keep = ~np.all(q.omega < 0, axis=1)  # drop the (-,-,-) octant
+    sub = _Subset(q, keep)
+    spec = P.batch_3d(n=5, block=2, quad=q)
+    spec.quad = sub
+    spec.sigma_s = np.concatenate([spec.sigma_s, 0.3 * spec.sigma_s], axis=1)
+    pb = oracle_problem(oracle, spec)
+    x = P.seeded_vector(spec.size, 77)
+    ref = pb.apply(x)
+    for precision in (ou.FP64, ou.FP32):
+        op = spec.operator(precision)
+        tol = FP64_TOL if precision == ou.FP64 else FP32_TOL
+        for shape in [(4, 1), (4, 4), (4, 1), (2, 2), (8, 3)]:
+            op.sweep_shape = shape
+            err = rel_err(op.Apply(x), ref)
+            assert err < tol, (precision, shape, err)
+        op.close()
+
+
+@pytest.mark.parametrize("solver", [ou.GMRES, ou.BICGSTAB, ou.SI])
+def test_device_nonfinite_ends_the_solve(solver):
+    """A value that turns non-finite on the device ends the solve with
+    OU_ERR_NONFINITE at once, not after max_it iterations.  Forced from the
+    host with finite inputs: in the FP32 sweep, material 1's scattering
+    source (sigma_s = 1e300 times M x) overflows FP32, and its inverse
+    (sigma_t = 1e300) underflows to zero, so psi = 0 x inf there."""
+    spec = P.uniform_2d(n=8)
+    spec.sigma_t = np.array([1.0, 1e300])
+    spec.sigma_s = np.array([[0.5], [1e300]])
+    spec.mat = (np.arange(64) % 2).astype(np.int32)
+    op = spec.operator(ou.FP32)
+    rhs = np.ones(spec.size)
+    with pytest.raises(ou.OuError) as e:
+        op.solve_rhs(rhs, solver=solver, tol=1e-10, max_it=100000)
+    assert e.value.code == -5, e.value
+    assert b"non-finite" in ou.lib().ou_last_error()

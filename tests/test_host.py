@@ -80,6 +80,59 @@ def test_real_harmonics_closed_forms():
     assert np.abs(Y[:, 6] - np.sqrt(5 / (16 * np.pi)) * (3 * o[:, 2] ** 2 - 1)).max() < 1e-14
 
 
+def test_real_harmonics_closed_forms_l3():
+    """Every l <= 3 harmonic against its Cartesian closed form (orthonormal,
+    Condon-Shortley phase), moment order per l: cos m = l..1, m = 0,
+    sin m = 1..l.  Written out with x = Omega_x, y = Omega_y, z = mu."""
+    q = ou.Quadrature(ou.PRODUCT_GLC, 3, n_polar=8, n_azim=16, L_max=3, weight_sum=4 * np.pi)
+    Y = q.M2D
+    x, y, z = q.omega[:, 0], q.omega[:, 1], q.omega[:, 2]
+    pi = np.pi
+    c1, c2a, c2b = np.sqrt(3 / (4 * pi)), np.sqrt(15 / (16 * pi)), np.sqrt(15 / (4 * pi))
+    c3a, c3b, c3c = np.sqrt(35 / (32 * pi)), np.sqrt(105 / (16 * pi)), np.sqrt(21 / (32 * pi))
+    closed = [
+        np.full_like(z, 1 / np.sqrt(4 * pi)),
+        -c1 * x, c1 * z, -c1 * y,
+        c2a * (x * x - y * y), -c2b * x * z, np.sqrt(5 / (16 * pi)) * (3 * z * z - 1),
+        -c2b * y * z, c2a * 2 * x * y,
+        -c3a * (x ** 3 - 3 * x * y * y), c3b * z * (x * x - y * y), -c3c * x * (5 * z * z - 1),
+        np.sqrt(7 / (16 * pi)) * (5 * z ** 3 - 3 * z),
+        -c3c * y * (5 * z * z - 1), c3b * z * 2 * x * y, -c3a * (3 * x * x * y - y ** 3),
+    ]
+    assert q.n_mom == 16 and list(q.mom_l) == [0] + [1] * 3 + [2] * 5 + [3] * 7
+    for m, ref in enumerate(closed):
+        assert np.abs(Y[:, m] - ref).max() < 2e-14, m
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("sn", [2, 4, 6, 8, 10, 12])
+def test_ls_tables_match_reference(dim, sn):
+    """The library's level-symmetric sets against LS.cc:14-169 as printed
+    (tests/golden/ls_tables.json): the first octant holds exactly the
+    reference's cosines in LS::compute_omega's triangle and the reference's
+    weight(i) up to the normalisation to sum 1 over all octants; every
+    direction is a unit vector and the weights integrate mu^2k exactly for
+    k <= sn/2."""
+    t = load_golden("ls_tables.json")[str(sn)]
+    cos = np.array(t["cosines"])
+    w_ref = np.array([t["classes"][k] for k in t["weights"]])
+    q = ou.Quadrature(ou.LS, dim, sn=sn, L_max=0)
+    n_oct = len(w_ref)
+    n_q = 4 if dim == 2 else 8
+    assert q.n_dir == n_q * n_oct
+    om, w = q.omega[:n_oct], q.weight[:n_oct]
+    assert (om > 0).all() and np.isin(om, cos).all()
+    row = 0
+    for j in range(len(cos)):
+        assert (om[row:row + len(cos) - j, 2] == cos[j]).all()
+        row += len(cos) - j
+    assert np.abs(w * n_q * w_ref.sum() - w_ref).max() < 1e-16
+    assert np.abs((q.omega ** 2).sum(axis=1) - 1.0).max() < 1e-15
+    assert abs(q.weight.sum() - 1.0) < 1e-15
+    for k in range(sn // 2 + 1):
+        assert abs((q.weight * q.omega[:, 2] ** (2 * k)).sum() - 1.0 / (2 * k + 1)) < 1e-15
+
+
 def test_reference_quadrature_fixture():
     fx = load_golden("quadrature_s4.json")
     ls = fx["ls_s4_galerkin"]
@@ -176,3 +229,35 @@ def test_missing_library_fails_loudly(tmp_path):
     finally:
         ou._lib, ou.LIB_PATH = saved_lib, saved_path
     assert ou.lib() is not None
+
+
+@pytest.mark.parametrize("dim,p,n,n_dir", [
+    (2, 1, (50000, 50000, 1), 40),     # 4 x 2.5e9 (quadrant, cell) pairs
+    (3, 2, (1, 2000, 2000), 288),      # 288 x 9 x 4e6 z-face trace values
+])
+def test_plan_rejects_meshes_past_32_bit_indices(dim, p, n, n_dir):
+    """Meshes whose (octant, cell) list or face-trace set would overflow the
+    32-bit indices of the schedule and kernels are refused with
+    OU_ERR_UNSUPPORTED before any array is read or any device memory is
+    touched (so this runs without a GPU; the arrays here are one entry
+    long and would be read past their end otherwise)."""
+    one = np.zeros(3 * n_dir)
+    ml = np.zeros(1, np.int32)
+    st = np.ones(1)
+    mat = np.zeros(1, np.int32)
+    ds = ou.ProblemDesc()
+    ds.dim, ds.fe_order = dim, p
+    for a in range(3):
+        ds.n_cells[a] = n[a]
+        ds.cell_size[a] = 1.0
+    ds.n_dir, ds.n_mom, ds.n_mat, ds.L = n_dir, 1, 1, 0
+    ds.omega = ds.M2D = ds.D2M = one.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    ds.mom_l = ml.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+    ds.sigma_t = ds.sigma_s = st.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    ds.material_id = mat.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+    ds.weight_sum = 1.0
+    plan = ctypes.c_void_p()
+    r = ou.lib().ou_plan_create(ctypes.byref(ds), ctypes.byref(plan))
+    assert r == -4, ou.lib().ou_last_error()
+    assert b"32-bit" in ou.lib().ou_last_error()
+    assert not plan.value

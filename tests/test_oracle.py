@@ -70,6 +70,61 @@ def test_quadrature_s4_fixture(oracle):
     assert np.abs(om[0] - np.array(g["omega0"])).max() < 1e-12
 
 
+LS_ORDERS = [2, 4, 6, 8, 10, 12]
+
+
+def _ls_fixture(sn):
+    t = load_golden("ls_tables.json")[str(sn)]
+    return np.array(t["cosines"]), np.array([t["classes"][k] for k in t["weights"]])
+
+
+@pytest.mark.parametrize("sn", LS_ORDERS)
+def test_ls_octant_against_reference_tables(oracle, sn):
+    """LS::build_octant (LS.cc:14-169), text-extracted: every cosine of the
+    oracle's octant is one of the printed direction[k] and every weight is
+    the printed weight(i), exactly and in the reference's order; the layout
+    is LS::compute_omega's triangle (omega_z constant over a row, omega_y
+    ascending along it)."""
+    cos, w_ref = _ls_fixture(sn)
+    om, w = oracle.ls_octant(sn)
+    n = len(cos)
+    assert om.shape[0] == n * (n + 1) // 2 == len(w_ref)
+    assert np.array_equal(w, w_ref)
+    assert np.isin(om, cos).all()
+    row = 0
+    for j in range(n):
+        blk = om[row:row + n - j]
+        assert (blk[:, 2] == cos[j]).all()
+        assert (np.diff(blk[:, 1]) > 0).all() and (np.diff(blk[:, 0]) < 0).all()
+        row += n - j
+
+
+@pytest.mark.parametrize("sn", LS_ORDERS)
+def test_ls_octant_mathematics(oracle, sn):
+    """What the level-symmetric construction fixes, independently of the
+    printed tables: unit directions (so a wrong cosine or a wrong index in
+    the triangle shows), the three axes alike, and the even polar moments
+    sum_d w_d mu_d^2k = sum_d w_d / (2k+1) exact for k <= sn/2 (a wrong
+    weight class or a class attached to the wrong direction shows)."""
+    om, w = oracle.ls_octant(sn)
+    assert np.abs((om ** 2).sum(axis=1) - 1.0).max() < 1e-15
+    W = w.sum()
+    assert abs(W - 1.0) < 1e-15
+    for a in range(3):
+        assert abs((w * om[:, a] ** 2).sum() - W / 3.0) < 1e-15
+        for k in range(sn // 2 + 1):
+            assert abs((w * om[:, a] ** (2 * k)).sum() - W / (2 * k + 1)) < 2e-16 * (k + 1)
+    # a plausible slip -- one weight class attached to its neighbour's
+    # direction -- breaks the moment conditions
+    if sn >= 6:
+        bad = w.copy()
+        i = int(np.argmax(w != w[0]))
+        bad[0], bad[i] = bad[i], bad[0]
+        errs = [abs((bad * om[:, a] ** (2 * k)).sum() - bad.sum() / (2 * k + 1))
+                for a in range(3) for k in range(sn // 2 + 1)]
+        assert max(errs) > 1e-6
+
+
 def test_gauss_legendre_against_reference_tables(oracle):
     """Newton Gauss-Legendre vs the tables of GLC.cc:60-175.  The S6 table is
     printed unsorted (0.661, 0.239, 0.932); compare as sets."""
