@@ -257,6 +257,18 @@ int ou_coarse_features(ou_plan *plan, int grid, const double *v, double *coarse,
  * every fine node.  z is a host vector of ou_domain_size doubles. */
 int ou_coarse_prolong(ou_plan *plan, int grid, const float *corners, double *z);
 
+/* ou_coarse_prolong on device arrays, allocating nothing.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops).  d_corners ([G^3][8] FP32)
+ * and d_z (ou_domain_size doubles, updated in place) are device pointers;
+ * the work is ordered on `stream` (NULL: the plan's stream) and not
+ * synchronised.  Its scratch (the (G+1)^3 vertex averages and the scale) is
+ * a fixed array of the library on each device, sized for grid <= 64;
+ * larger grids return OU_ERR_SIZE.  Calls that overlap on one device share
+ * that array and race: order them on one stream. */
+int ou_coarse_prolong_noalloc(ou_plan *plan, int grid, const float *d_corners, double *d_z,
+                              void *stream);
+
 /* Training pairs from the operator's own error propagation: starting from
  * e_0 (host vector, or NULL for e_0 = the source-driven right-hand side
  * D L^-1 (q + inflow)), runs e_{k+1} = e_k - A e_k for n_steps steps on the

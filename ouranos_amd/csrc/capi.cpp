@@ -874,6 +874,19 @@ int ou_coarse_prolong(ou_plan *plan, int grid, const float *corners, double *z) 
   });
 }
 
+int ou_coarse_prolong_noalloc(ou_plan *plan, int grid, const float *d_corners, double *d_z,
+                              void *stream) {
+  return guarded([&] {
+    ou::Plan &pl = plan_of(plan);
+    if (grid > ou::coarse_noalloc_max_grid())
+      ou::fail(OU_ERR_SIZE, "coarse grid above the allocation-free prolongation's maximum");
+    ou::coarse_factor(pl, grid);
+    if (!d_corners || !d_z) ou::fail(OU_ERR_ARG, "null argument");
+    ou::coarse_prolong_add_noalloc(pl, grid, d_corners, d_z, stream_of(plan, stream));
+    return int(OU_OK);
+  });
+}
+
 int ou_learned_pairs(ou_plan *plan, int grid, const double *e0, int n_steps,
                      unsigned long long record_mask, double *r_out, double *e_out) {
   return guarded([&] {

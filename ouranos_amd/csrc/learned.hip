@@ -20,10 +20,23 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <mutex>
 #include <string>
 
 #include "learned.hpp"
 #include "vec.hpp"
+
+// Scratch of the allocation-free prolongation (coarse_prolong_add_noalloc):
+// the (G+1)^3 vertex averages and the scale, for coarse grids up to
+// kNoAllocMaxGrid.  Each device holds its own copy until the process ends;
+// host code reaches it only through the address hipGetSymbolAddress gives
+// per device (the array's name is a host-side stand-in).
+namespace {
+constexpr int kNoAllocMaxGrid = 64;
+constexpr long kNoAllocScratchDoubles =
+    long(kNoAllocMaxGrid + 1) * (kNoAllocMaxGrid + 1) * (kNoAllocMaxGrid + 1) + 1;
+}  // namespace
+static __device__ __align__(256) double scratch[kNoAllocScratchDoubles];
 
 namespace ou {
 
@@ -216,6 +229,56 @@ void coarse_prolong_add(const Plan &pl, int G, const float *d_corners, const dou
   k_corner_to_vertex<<<blocks(V), 256, 0, s>>>(d_corners, G, d_vertex);
   k_prolong_vertex<<<blocks(pl.n_dofs), 256, 0, s>>>(d_z, d_vertex, d_dot, long(G) * G * G, t,
                                                      pl.n_dofs);
+  OU_HIP(hipGetLastError());
+}
+
+namespace {
+
+__global__ void k_set_scalar(double *dst, double v) { *dst = v; }
+
+// The device address of `scratch` on the current device, looked up once per
+// device (with that device current) and kept.
+double *noalloc_scratch() {
+  constexpr int kMaxDevices = 64;
+  static std::mutex mu;
+  static double *addr[kMaxDevices] = {};
+  int dev = 0;
+  OU_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) fail(OU_ERR_UNSUPPORTED, "device ordinal above 63");
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      (void)hipGetLastError();
+      fail(OU_ERR_HIP, std::string("hipGetSymbolAddress of the prolongation scratch failed: ") +
+                           hipGetErrorString(e));
+    }
+    addr[dev] = static_cast<double *>(p);
+  }
+  return addr[dev];
+}
+
+}  // namespace
+
+int coarse_noalloc_max_grid() { return kNoAllocMaxGrid; }
+
+void coarse_prolong_add_noalloc(const Plan &pl, int G, const float *d_corners, double *d_z,
+                                hipStream_t s) {
+  if (G > kNoAllocMaxGrid)
+    fail(OU_ERR_SIZE, "coarse grid " + std::to_string(G) +
+                          " is above the allocation-free prolongation's maximum of " +
+                          std::to_string(kNoAllocMaxGrid));
+  const Transfer t = transfer_of(pl, G);
+  double *scr = noalloc_scratch();
+  const long V = long(G + 1) * (G + 1) * (G + 1);
+  const long P0 = long(G) * G * G;
+  double *d_vertex = scr, *d_dot = scr + V;
+  // scale s = sqrt(dot / (8 G^3)) = 1, as the allocating entry point sets it;
+  // every vertex value is written by k_corner_to_vertex before it is read
+  k_set_scalar<<<1, 1, 0, s>>>(d_dot, 8.0 * double(P0));
+  k_corner_to_vertex<<<blocks(V), 256, 0, s>>>(d_corners, G, d_vertex);
+  k_prolong_vertex<<<blocks(pl.n_dofs), 256, 0, s>>>(d_z, d_vertex, d_dot, P0, t, pl.n_dofs);
   OU_HIP(hipGetLastError());
 }
 

@@ -35,6 +35,14 @@ void coarse_restrict(const Plan &pl, int G, const double *d_v, double *d_coarse,
 void coarse_prolong_add(const Plan &pl, int G, const float *d_corners, const double *d_dot,
                         double *d_vertex, double *d_z, hipStream_t s);
 
+// The same with s = 1, allocating nothing: the vertex averages and the scale
+// live in a fixed device array of the library (grids up to
+// coarse_noalloc_max_grid(), else OU_ERR_SIZE).  Stream-ordered; calls that
+// overlap on one device share that array and race.
+int coarse_noalloc_max_grid();
+void coarse_prolong_add_noalloc(const Plan &pl, int G, const float *d_corners, double *d_z,
+                                hipStream_t s);
+
 // Right preconditioner z = r + P s N_odd(R r / s) on moment 0:
 // R = coarse_restrict (8 octant values per coarse cell), s = RMS of R r,
 // N_odd the network's odd prediction of the correction beyond the identity
