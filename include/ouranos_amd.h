@@ -34,6 +34,8 @@ typedef enum {
 } ou_status;
 
 typedef enum { OU_FP64 = 0, OU_FP32 = 1 } ou_precision;
+/* How a sweep is launched (ou_set_sweep_mode). */
+typedef enum { OU_SWEEP_LEVELS = 0, OU_SWEEP_PERSISTENT = 1 } ou_sweep_mode_kind;
 /* OU_BC_REFLECTIVE is what a parameter file may name (Parameters.cc:59-70);
  * the reference never implements it in the sweep, and plans reject it with
  * OU_ERR_UNSUPPORTED. */
@@ -340,6 +342,31 @@ int ou_sweep_shape(const ou_plan *plan, int *waves, int *dir_groups);
 int ou_set_sweep_streams(ou_plan *plan, int n_streams);
 int ou_sweep_streams(const ou_plan *plan, int *n_streams);
 
+/* How a sweep is launched.  OU_SWEEP_LEVELS: one launch per wavefront level
+ * (octant parts per ou_set_sweep_streams).  OU_SWEEP_PERSISTENT: one launch
+ * for the whole sweep; resident workgroups take (chunk, direction group)
+ * items in level order and start one as soon as the upwind level of its own
+ * octant and group is done, the dataflow of the reference's scheduler
+ * (RadiativeTransfer.cc:605-778, Task.cc:52-62) with the level as the unit
+ * of readiness.  Both modes run the same per-chunk arithmetic: results are
+ * bit-for-bit the same.  A new plan picks the mode its workload runs faster
+ * in. */
+int ou_set_sweep_mode(ou_plan *plan, int mode);
+int ou_sweep_mode(const ou_plan *plan, int *mode);
+
+/* Watchdog of the persistent sweep: a workgroup that waits longer than
+ * `seconds` for its upwind level gives up, every workgroup of the launch
+ * leaves, the operator's output is NaN and the call that synchronises next
+ * (ou_apply, ou_sweep, ou_solve*, ou_sweep_status) returns OU_ERR_TIMEOUT.
+ * The reference has no such bound: a missing message makes get_next_task
+ * spin forever (RadiativeTransfer.cc:620-623).  Default 10 s; 0 gives up at
+ * the first wait that finds its upwind level unfinished. */
+int ou_set_sweep_wait_limit(ou_plan *plan, double seconds);
+/* OU_ERR_TIMEOUT (and clears the condition) if a persistent sweep of the
+ * plan gave up since the last check, else OU_OK.  Synchronises the plan's
+ * stream. */
+int ou_sweep_status(ou_plan *plan);
+
 typedef struct ou_batch ou_batch;
 
 typedef struct {
@@ -371,6 +398,11 @@ int ou_batch_set_graph(ou_batch *batch, int use_graph);
  * graph as well); 1 launches each level as one kernel.  Results are
  * bit-for-bit the same. */
 int ou_batch_set_streams(ou_batch *batch, int n_streams);
+/* OU_SWEEP_PERSISTENT sweeps every member's levels in one persistent launch
+ * (inside the captured graph); OU_SWEEP_LEVELS launches the merged levels.
+ * Bit-for-bit the same results.  A watchdog trip (the first member's wait
+ * limit) makes the next synchronising batched call return OU_ERR_TIMEOUT. */
+int ou_batch_set_sweep_mode(ou_batch *batch, int mode);
 
 /* Apply of every member: d_y[i] = d_x[i] - D L^-1 M S d_x[i] (device vectors;
  * d_x / d_y are host arrays of n_plans device pointers; all d_y distinct and

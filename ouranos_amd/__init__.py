@@ -30,6 +30,7 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libouranos_
 FP64, FP32 = 0, 1
 LS, GLC, PRODUCT_GLC = 0, 1, 2
 SI, GMRES, BICGSTAB, FGMRES = 0, 1, 2, 3
+SWEEP_LEVELS, SWEEP_PERSISTENT = 0, 1
 BC_VACUUM, BC_ISOTROPIC, BC_MOST_NORMAL = 0, 1, 2
 
 STATUS = {0: "OK", -1: "ARG", -2: "SIZE", -3: "HIP", -4: "UNSUPPORTED", -5: "NONFINITE",
@@ -145,6 +146,11 @@ def lib():
         L.ou_set_sweep_shape.argtypes = [_vp, ctypes.c_int, ctypes.c_int]
         L.ou_sweep_shape.argtypes = [_vp, _ip, _ip]
         L.ou_set_sweep_streams.argtypes = [_vp, ctypes.c_int]
+        L.ou_set_sweep_mode.argtypes = [_vp, ctypes.c_int]
+        L.ou_sweep_mode.argtypes = [_vp, _ip]
+        L.ou_set_sweep_wait_limit.argtypes = [_vp, ctypes.c_double]
+        L.ou_sweep_status.argtypes = [_vp]
+        L.ou_batch_set_sweep_mode.argtypes = [_vp, ctypes.c_int]
         L.ou_sweep_streams.argtypes = [_vp, _ip]
         L.ou_batch_create.argtypes = [ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int,
                                       ctypes.c_int, ctypes.POINTER(_vp)]
@@ -524,6 +530,28 @@ class RadiativeTransfer:
     def sweep_streams(self, n):
         _check(lib().ou_set_sweep_streams(self._plan, int(n)))
 
+    @property
+    def sweep_mode(self):
+        """SWEEP_LEVELS (one launch per wavefront level) or SWEEP_PERSISTENT
+        (one persistent launch per sweep)."""
+        m = ctypes.c_int()
+        _check(lib().ou_sweep_mode(self._plan, ctypes.byref(m)))
+        return m.value
+
+    @sweep_mode.setter
+    def sweep_mode(self, mode):
+        _check(lib().ou_set_sweep_mode(self._plan, int(mode)))
+
+    def set_sweep_wait_limit(self, seconds):
+        """Watchdog of the persistent sweep: a wait for an upwind level gives
+        up after `seconds` (0: at the first unfinished check)."""
+        _check(lib().ou_set_sweep_wait_limit(self._plan, ctypes.c_double(seconds)))
+
+    def sweep_status(self):
+        """Raises OuError(OU_ERR_TIMEOUT) if a persistent sweep gave up since
+        the last check."""
+        _check(lib().ou_sweep_status(self._plan))
+
     # --- device API --------------------------------------------------------------
     def apply_device(self, d_x, d_y):
         _check(lib().ou_apply_device(self._plan, d_x.ptr, d_y.ptr, None))
@@ -584,6 +612,10 @@ class Batch:
     def set_streams(self, n):
         """Launches per merged level: 2 (octant halves on two streams) or 1."""
         _check(lib().ou_batch_set_streams(self._b, int(n)))
+
+    def set_sweep_mode(self, mode):
+        """SWEEP_PERSISTENT: every member's levels in one persistent launch."""
+        _check(lib().ou_batch_set_sweep_mode(self._b, int(mode)))
 
     def apply_device(self, xs, ys, stream=None):
         _check(lib().ou_batch_apply_device(self._b, _ptr_table(xs), _ptr_table(ys), stream))

@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -125,6 +126,11 @@ Batch::~Batch() { release(); }
 
 void Batch::release() {
   drop_graphs();
+  for (void *p : {static_cast<void *>(d_dep_off_), static_cast<void *>(d_dep_list_),
+                  static_cast<void *>(d_done_), static_cast<void *>(d_error_)})
+    if (p) (void)hipFree(p);
+  d_dep_off_ = d_dep_list_ = d_error_ = nullptr;
+  d_done_ = nullptr;
   for (void *p : {static_cast<void *>(d_chunks_), static_cast<void *>(d_chunk_view_),
                   static_cast<void *>(d_views_), static_cast<void *>(d_members_),
                   static_cast<void *>(d_io_), static_cast<void *>(d_si_work_),
@@ -184,7 +190,7 @@ void Batch::build_schedule() {
                        hipMemcpyDeviceToHost));
   }
   std::vector<Chunk> chunks;
-  std::vector<int> view;
+  std::vector<int> view, local_index;
   bl_.level_off.assign(1, 0);
   bl_.level_split.assign(size_t(n_levels), 0);
   for (int l = 0; l < n_levels; ++l) {
@@ -197,6 +203,7 @@ void Batch::build_schedule() {
         for (int c = pl.quarter_begin(l, 2 * half); c < pl.quarter_begin(l, 2 * half + 2); ++c) {
           chunks.push_back(host[u][size_t(c)]);
           view.push_back(int(u));
+          local_index.push_back(c);
         }
       }
     }
@@ -207,6 +214,42 @@ void Batch::build_schedule() {
   d_chunk_view_ = dev_upload(view);
   bl_.chunks = d_chunks_;
   bl_.chunk_view = d_chunk_view_;
+  // The persistent launch's dependency lists: each member's own, with its
+  // chunks renumbered into the merged list.
+  {
+    if (long(chunks.size()) * kMaxDirGroups > long(INT32_MAX))
+      fail(OU_ERR_UNSUPPORTED, "too many chunks for the batch's persistent launch");
+    std::vector<std::vector<int>> merged_of(members_.size());
+    for (size_t u = 0; u < members_.size(); ++u)
+      merged_of[u].assign(size_t(members_[u]->plan().n_chunks), -1);
+    for (size_t i = 0; i < chunks.size(); ++i)
+      merged_of[size_t(view[i])][size_t(local_index[i])] = int(i);
+    std::vector<int> dep_off(chunks.size() + 1, 0), dep_list;
+    for (size_t i = 0; i < chunks.size(); ++i) {
+      const PersistTables &t = members_[size_t(view[i])]->plan().deps;
+      const int c = local_index[i];
+      for (int k = t.dep_off[size_t(c)]; k < t.dep_off[size_t(c) + 1]; ++k)
+        dep_list.push_back(merged_of[size_t(view[i])][size_t(t.dep_list[size_t(k)])]);
+      dep_off[i + 1] = int(dep_list.size());
+    }
+    d_dep_off_ = dev_upload(dep_off);
+    d_dep_list_ = dev_upload(dep_list);
+    const size_t nd = chunks.size() * kMaxDirGroups + 1;
+    OU_HIP(hipMalloc(reinterpret_cast<void **>(&d_done_), nd * sizeof(unsigned)));
+    OU_HIP(hipMemset(d_done_, 0, nd * sizeof(unsigned)));
+    OU_HIP(hipMalloc(reinterpret_cast<void **>(&d_error_), sizeof(int)));
+    OU_HIP(hipMemset(d_error_, 0, sizeof(int)));
+    bl_.cus = p0.cus;
+    PersistView &pv = bl_.persist;
+    pv.dep_off = d_dep_off_;
+    pv.dep_list = d_dep_list_;
+    pv.done = d_done_;
+    pv.ticket = d_done_ + chunks.size() * kMaxDirGroups;
+    pv.error = d_error_;
+    pv.wait_ticks =
+        (unsigned long long)(std::max(0.0, p0.wait_limit_s) * p0.wall_clock_khz * 1000.0);
+    pv.max_polls = wait_polls(p0.wait_limit_s);
+  }
   bl_.waves = p0.waves;
   bl_.dir_groups = p0.dir_groups;
 }
@@ -272,6 +315,18 @@ void Batch::set_shape(int waves, int dir_groups) {
   batch_prepare(bl_);
   drop_graphs();
   build_views();
+}
+
+void Batch::set_sweep_mode(int mode) {
+  if (mode != OU_SWEEP_LEVELS && mode != OU_SWEEP_PERSISTENT)
+    fail(OU_ERR_ARG, "sweep mode must be OU_SWEEP_LEVELS or OU_SWEEP_PERSISTENT");
+  OU_HIP(hipStreamSynchronize(stream_));
+  bl_.sweep_mode = mode;
+  drop_graphs();
+}
+
+void Batch::check_watchdog(hipStream_t s) {
+  if (bl_.sweep_mode == OU_SWEEP_PERSISTENT) check_sweep_watchdog(d_error_, s ? s : stream_);
 }
 
 void Batch::set_streams(int n) {
@@ -403,6 +458,7 @@ int Batch::solve_si(double *const *d_phi, double tol, int max_it, int *iteration
       OU_HIP(hipMemcpyAsync(h_si_norms_, d_si_norms_, n * 2 * sizeof(double),
                             hipMemcpyDeviceToHost, stream_));
       OU_HIP(hipStreamSynchronize(stream_));
+      check_watchdog(stream_);
       ++it;
       size_t done = 0;
       double worst = 0.0;
@@ -460,7 +516,9 @@ std::string Batch::tune_key() const {
 std::vector<TuneEntry> Batch::tune(int reps) {
   if (reps < 1) reps = 1;
   const Plan &p0 = members_[0]->plan();
-  const int max_g = std::min(4, std::max(1, max_dirs_per_octant(p0)));
+  int min_g = 1;
+  for (const RadiativeTransfer *m : members_) min_g = std::max(min_g, m->plan().min_dir_groups);
+  const int max_g = std::max(min_g, std::min(4, std::max(1, max_dirs_per_octant(p0))));
   // Scratch vectors: one shared input, one output per member.
   long max_vec = 0, sum_vec = 0;
   for (const RadiativeTransfer *m : members_) {
@@ -486,7 +544,7 @@ std::vector<TuneEntry> Batch::tune(int reps) {
     OU_HIP(hipEventCreate(&e1));
     for (int w : {2, 4, 6, 8, 12}) {
       if (!sweep_supported(p0.dim, p0.p, p0.mm, p0.precision, w, p0.max_lds)) continue;
-      for (int g = 1; g <= max_g; ++g) {
+      for (int g = min_g; g <= max_g; ++g) {
         set_shape(w, g);
         apply(xs.data(), ys.data(), stream_);  // warm-up, captures the graph
         OU_HIP(hipStreamSynchronize(stream_));

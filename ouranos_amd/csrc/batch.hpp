@@ -59,6 +59,12 @@ class Batch {
                bool verbose);
 
   void set_shape(int waves, int dir_groups);
+  // OU_SWEEP_LEVELS or OU_SWEEP_PERSISTENT (one persistent launch).
+  void set_sweep_mode(int mode);
+  int sweep_mode() const { return bl_.sweep_mode; }
+  // OU_ERR_TIMEOUT if a persistent-sweep wait gave up since the last check
+  // (synchronises s, or the batch's stream).
+  void check_watchdog(hipStream_t s);
   void set_use_graph(bool on) { use_graph_ = on; }
   // 2: each level's second octant half on a side stream (the default); 1: one
   // launch per level.
@@ -88,6 +94,8 @@ class Batch {
   hipStream_t side_ = nullptr;  // the second octant half's level launches
   hipEvent_t ev_fork_ = nullptr, ev_join_ = nullptr;
   Chunk *d_chunks_ = nullptr;
+  int *d_dep_off_ = nullptr, *d_dep_list_ = nullptr, *d_error_ = nullptr;
+  unsigned *d_done_ = nullptr;  // persistent launch: completion flags, then the ticket
   int *d_chunk_view_ = nullptr;
   SweepView *d_views_ = nullptr;     // [2][members]: plain, with source
   MemberView *d_members_ = nullptr;  // [members]

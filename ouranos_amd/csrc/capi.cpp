@@ -2,6 +2,7 @@
 // this boundary: every call returns OU_OK or a negative ou_status and keeps
 // the message for ou_last_error().
 #include <algorithm>
+#include <initializer_list>
 #include <chrono>
 #include <cstring>
 #include <string>
@@ -80,6 +81,27 @@ hipStream_t stream_of(ou_plan *p, void *s) {
 void ensure_staging(ou::Plan &pl) {
   if (!pl.d_x) OU_HIP(hipMalloc(&pl.d_x, size_t(pl.vec_size) * 8));
   if (!pl.d_y) OU_HIP(hipMalloc(&pl.d_y, size_t(pl.vec_size) * 8));
+}
+
+// OU_ERR_TIMEOUT if the plan's persistent sweep gave up since the last check.
+void check_watchdog(ou::Plan &pl) {
+  if (pl.sweep_mode == OU_SWEEP_PERSISTENT) ou::check_sweep_watchdog(pl.d_error, pl.stream);
+}
+
+// Runs a solve; a watchdog trip of the persistent sweep makes the operator's
+// output NaN, which the solver reports as non-finite: that is turned back into
+// the watchdog's OU_ERR_TIMEOUT here, and any trip is reported after the solve.
+template <typename F>
+auto with_watchdog(std::initializer_list<ou::Plan *> plans, F &&f) -> decltype(f()) {
+  try {
+    auto r = f();
+    for (ou::Plan *p : plans) check_watchdog(*p);
+    return r;
+  } catch (const ou::Error &e) {
+    if (e.code == OU_ERR_NONFINITE)
+      for (ou::Plan *p : plans) check_watchdog(*p);
+    throw;
+  }
 }
 
 void check_finite_host(const double *x, long n) {
@@ -213,6 +235,7 @@ int ou_apply(ou_plan *plan, const double *x, double *y) {
     plan->rt->Apply(pl.d_x, pl.d_y, pl.stream);
     OU_HIP(hipMemcpyAsync(y, pl.d_y, bytes, hipMemcpyDeviceToHost, pl.stream));
     OU_HIP(hipStreamSynchronize(pl.stream));
+    check_watchdog(pl);
     return int(OU_OK);
   });
 }
@@ -231,6 +254,7 @@ int ou_apply_multi(ou_plan *plan, const double *x, double *y, int n_rhs) {
       OU_HIP(hipMemcpyAsync(y + r * n, pl.d_y, bytes, hipMemcpyDeviceToHost, pl.stream));
     }
     OU_HIP(hipStreamSynchronize(pl.stream));
+    check_watchdog(pl);
     return int(OU_OK);
   });
 }
@@ -277,6 +301,7 @@ int ou_sweep(ou_plan *plan, double *y, int with_source) {
     plan->rt->sweep(pl.d_y, with_source != 0, pl.stream);
     OU_HIP(hipMemcpyAsync(y, pl.d_y, bytes, hipMemcpyDeviceToHost, pl.stream));
     OU_HIP(hipStreamSynchronize(pl.stream));
+    check_watchdog(pl);
     return int(OU_OK);
   });
 }
@@ -332,25 +357,24 @@ int ou_solve(ou_plan *plan, const ou_solver_opts *opts, double *phi, ou_solve_in
     const auto t0 = std::chrono::steady_clock::now();
     ou::SolveResult r;
     hipStream_t s = pl.stream;
-    switch (opts->solver) {
-      case OU_SOLVER_SI:
-        r = ou::solve_si(rt, d_phi, opts->tol, opts->max_it, opts->verbose != 0);
-        break;
-      case OU_SOLVER_GMRES:
-      case OU_SOLVER_FGMRES:
-      case OU_SOLVER_BICGSTAB:
-        OU_HIP(hipMalloc(&d_rhs, bytes));
-        ou::build_rhs(rt, d_rhs, s);
-        if (opts->solver == OU_SOLVER_BICGSTAB)
-          r = ou::solve_bicgstab(rt, d_rhs, d_phi, opts->tol, opts->max_it,
-                                 opts->verbose != 0, s);
-        else
-          r = ou::solve_gmres(rt, d_rhs, d_phi, opts->restart, opts->tol, opts->max_it,
-                              nullptr, opts->verbose != 0, s);
-        break;
-      default:
-        ou::fail(OU_ERR_ARG, "unknown solver");
-    }
+    r = with_watchdog({&pl}, [&] {
+      switch (opts->solver) {
+        case OU_SOLVER_SI:
+          return ou::solve_si(rt, d_phi, opts->tol, opts->max_it, opts->verbose != 0);
+        case OU_SOLVER_GMRES:
+        case OU_SOLVER_FGMRES:
+        case OU_SOLVER_BICGSTAB:
+          OU_HIP(hipMalloc(&d_rhs, bytes));
+          ou::build_rhs(rt, d_rhs, s);
+          if (opts->solver == OU_SOLVER_BICGSTAB)
+            return ou::solve_bicgstab(rt, d_rhs, d_phi, opts->tol, opts->max_it,
+                                      opts->verbose != 0, s);
+          return ou::solve_gmres(rt, d_rhs, d_phi, opts->restart, opts->tol, opts->max_it,
+                                 nullptr, opts->verbose != 0, s);
+        default:
+          ou::fail(OU_ERR_ARG, "unknown solver");
+      }
+    });
     const double secs =
         std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     OU_HIP(hipMemcpy(phi, d_phi, bytes, hipMemcpyDeviceToHost));
@@ -414,21 +438,22 @@ int ou_solve_rhs(ou_plan *plan, const ou_solver_opts *opts, const double *rhs, d
     const auto t0 = std::chrono::steady_clock::now();
     hipStream_t s = pl.stream;
     ou::SolveResult r;
-    switch (opts->solver) {
-      case OU_SOLVER_SI:
-        r = ou::solve_si_rhs(rt, d_rhs, d_phi, opts->tol, opts->max_it, opts->verbose != 0, s);
-        break;
-      case OU_SOLVER_BICGSTAB:
-        r = ou::solve_bicgstab(rt, d_rhs, d_phi, opts->tol, opts->max_it, opts->verbose != 0, s);
-        break;
-      case OU_SOLVER_GMRES:
-      case OU_SOLVER_FGMRES:
-        r = ou::solve_gmres(rt, d_rhs, d_phi, opts->restart, opts->tol, opts->max_it, nullptr,
-                            opts->verbose != 0, s);
-        break;
-      default:
-        ou::fail(OU_ERR_ARG, "unknown solver");
-    }
+    r = with_watchdog({&pl}, [&] {
+      switch (opts->solver) {
+        case OU_SOLVER_SI:
+          return ou::solve_si_rhs(rt, d_rhs, d_phi, opts->tol, opts->max_it,
+                                  opts->verbose != 0, s);
+        case OU_SOLVER_BICGSTAB:
+          return ou::solve_bicgstab(rt, d_rhs, d_phi, opts->tol, opts->max_it,
+                                    opts->verbose != 0, s);
+        case OU_SOLVER_GMRES:
+        case OU_SOLVER_FGMRES:
+          return ou::solve_gmres(rt, d_rhs, d_phi, opts->restart, opts->tol, opts->max_it,
+                                 nullptr, opts->verbose != 0, s);
+        default:
+          ou::fail(OU_ERR_ARG, "unknown solver");
+      }
+    });
     const double secs =
         std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     OU_HIP(hipMemcpy(phi, d_phi, bytes, hipMemcpyDeviceToHost));
@@ -532,11 +557,15 @@ int ou_time_apply(ou_plan *plan, const double *d_x, double *d_y, int warmup, int
       OU_HIP(hipEventElapsedTime(&ms, pl.ev[1], pl.ev[2]));
       ms_sweep += ms;
     }
+    check_watchdog(pl);
     int launches = 0;
     const int ns = pl.sweep_streams;
-    for (int l = 0; l < pl.n_levels; ++l)
-      for (int k = 0; k < ns; ++k)
-        launches += pl.quarter_begin(l, 4 * (k + 1) / ns) > pl.quarter_begin(l, 4 * k / ns);
+    if (pl.sweep_mode == OU_SWEEP_PERSISTENT)
+      launches = 2;  // counter reset + the persistent sweep
+    else
+      for (int l = 0; l < pl.n_levels; ++l)
+        for (int k = 0; k < ns; ++k)
+          launches += pl.quarter_begin(l, 4 * (k + 1) / ns) > pl.quarter_begin(l, 4 * k / ns);
     timing->ms_total = ms_total;
     timing->ms_sweep_kernel = ms_sweep;
     timing->ms_other = ms_total - ms_sweep;
@@ -581,6 +610,51 @@ int ou_sweep_streams(const ou_plan *plan, int *n_streams) {
   return guarded([&] {
     if (!n_streams) ou::fail(OU_ERR_ARG, "null output pointer");
     *n_streams = plan_of(const_cast<ou_plan *>(plan)).sweep_streams;
+    return int(OU_OK);
+  });
+}
+
+int ou_set_sweep_mode(ou_plan *plan, int mode) {
+  return guarded([&] {
+    ou::Plan &pl = plan_of(plan);
+    if (mode != OU_SWEEP_LEVELS && mode != OU_SWEEP_PERSISTENT)
+      ou::fail(OU_ERR_ARG, "sweep mode must be OU_SWEEP_LEVELS or OU_SWEEP_PERSISTENT");
+    OU_HIP(hipStreamSynchronize(pl.stream));
+    pl.sweep_mode = mode;
+    return int(OU_OK);
+  });
+}
+
+int ou_sweep_mode(const ou_plan *plan, int *mode) {
+  return guarded([&] {
+    if (!mode) ou::fail(OU_ERR_ARG, "null output pointer");
+    *mode = plan_of(const_cast<ou_plan *>(plan)).sweep_mode;
+    return int(OU_OK);
+  });
+}
+
+int ou_set_sweep_wait_limit(ou_plan *plan, double seconds) {
+  return guarded([&] {
+    ou::Plan &pl = plan_of(plan);
+    if (!(seconds >= 0.0) || seconds > 3600.0)
+      ou::fail(OU_ERR_ARG, "wait limit must be in [0, 3600] seconds");
+    OU_HIP(hipStreamSynchronize(pl.stream));
+    pl.wait_limit_s = seconds;
+    return int(OU_OK);
+  });
+}
+
+int ou_sweep_status(ou_plan *plan) {
+  return guarded([&] {
+    ou::Plan &pl = plan_of(plan);
+    ou::check_sweep_watchdog(pl.d_error, pl.stream);
+    return int(OU_OK);
+  });
+}
+
+int ou_batch_set_sweep_mode(ou_batch *batch, int mode) {
+  return guarded([&] {
+    batch_of(batch).set_sweep_mode(mode);
     return int(OU_OK);
   });
 }
@@ -702,6 +776,7 @@ int ou_batch_time_apply(ou_batch *batch, const double *const *d_x, double *const
     for (int i = 0; i < steps; ++i) b.apply(d_x, d_y, s);
     OU_HIP(hipEventRecord(e1, s));
     OU_HIP(hipEventSynchronize(e1));
+    b.check_watchdog(s);
     float t = 0.0f;
     OU_HIP(hipEventElapsedTime(&t, e0, e1));
     *ms = double(t) / steps;
@@ -1056,9 +1131,11 @@ int ou_solve_learned(ou_plan *plan, ou_unet *net, const ou_solver_opts *opts, do
     OU_HIP(hipMemcpy(d_phi, phi, bytes, hipMemcpyHostToDevice));
     hipStream_t s = pl.stream;
     const auto t0 = std::chrono::steady_clock::now();
-    ou::build_rhs(rt, d_rhs, s);
-    const ou::SolveResult r = ou::solve_gmres(rt, d_rhs, d_phi, opts->restart, opts->tol,
-                                              opts->max_it, prec.get(), opts->verbose != 0, s);
+    const ou::SolveResult r = with_watchdog({&pl}, [&] {
+      ou::build_rhs(rt, d_rhs, s);
+      return ou::solve_gmres(rt, d_rhs, d_phi, opts->restart, opts->tol, opts->max_it,
+                             prec.get(), opts->verbose != 0, s);
+    });
     const double secs =
         std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     OU_HIP(hipMemcpy(phi, d_phi, bytes, hipMemcpyDeviceToHost));
@@ -1072,7 +1149,7 @@ int ou_solve_refined(ou_plan *plan64, ou_plan *plan32, ou_unet *net, const ou_so
                      double inner_tol, double *phi, ou_solve_info *info) {
   return guarded([&] {
     ou::Plan &p64 = plan_of(plan64);
-    plan_of(plan32);
+    ou::Plan &p32 = plan_of(plan32);
     if (!opts || !phi) ou::fail(OU_ERR_ARG, "null argument");
     if (!(opts->tol > 0.0) || opts->max_it < 1 || !(inner_tol > 0.0) || !(inner_tol < 1.0))
       ou::fail(OU_ERR_ARG, "bad tolerance or max_it");
@@ -1091,9 +1168,10 @@ int ou_solve_refined(ou_plan *plan64, ou_plan *plan32, ou_unet *net, const ou_so
     } guard{&d_phi};
     OU_HIP(hipMemcpy(d_phi, phi, bytes, hipMemcpyHostToDevice));
     const auto t0 = std::chrono::steady_clock::now();
-    const ou::RefinedResult r =
-        ou::solve_refined(*plan64->rt, *plan32->rt, prec.get(), d_phi, opts->tol, opts->max_it,
-                          opts->restart, inner_tol, opts->verbose != 0);
+    const ou::RefinedResult r = with_watchdog({&p64, &p32}, [&] {
+      return ou::solve_refined(*plan64->rt, *plan32->rt, prec.get(), d_phi, opts->tol,
+                               opts->max_it, opts->restart, inner_tol, opts->verbose != 0);
+    });
     const double secs =
         std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     OU_HIP(hipMemcpy(phi, d_phi, bytes, hipMemcpyDeviceToHost));

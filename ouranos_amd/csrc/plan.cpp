@@ -17,6 +17,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <numeric>
+#include <string>
 
 #include "fe.hpp"
 #include "plan.hpp"
@@ -65,12 +66,95 @@ bool finite_all(const double *p, size_t n) {
 
 }  // namespace
 
+PersistTables chunk_dependencies(const std::vector<Chunk> &chunks, const std::vector<int> &cells,
+                                 const int n[3], int dim) {
+  const int nx = n[0], ny = n[1], nz = n[2];
+  std::vector<std::vector<int>> by_oct(8);
+  for (size_t c = 0; c < chunks.size(); ++c) by_oct[size_t(chunks[c].octant)].push_back(int(c));
+  std::vector<int> owner(size_t(nx) * ny * nz, -1);
+  std::vector<std::vector<int>> deps(chunks.size());
+  for (int o = 0; o < 8; ++o) {
+    if (by_oct[size_t(o)].empty()) continue;
+    auto cells_of = [&](int c) {
+      const Chunk &ch = chunks[size_t(c)];
+      return std::make_pair(ch.cell_offset, ch.cell_offset + ch.count_lo + ch.count_hi);
+    };
+    for (int c : by_oct[size_t(o)]) {
+      const auto r = cells_of(c);
+      for (int q = r.first; q < r.second; ++q) owner[size_t(cells[size_t(q)])] = c;
+    }
+    for (int c : by_oct[size_t(o)]) {
+      std::vector<int> &d = deps[size_t(c)];
+      const auto r = cells_of(c);
+      for (int q = r.first; q < r.second; ++q) {
+        const long cell = cells[size_t(q)];
+        const int ijk[3] = {int(cell % nx), int((cell / nx) % ny), int(cell / (long(nx) * ny))};
+        for (int a = 0; a < dim; ++a) {
+          // one step upwind: towards +a where Omega_a < 0 (octant bit a set)
+          int nb[3] = {ijk[0], ijk[1], ijk[2]};
+          nb[a] += (o >> a) & 1 ? 1 : -1;
+          if (nb[a] < 0 || nb[a] >= n[a]) continue;
+          d.push_back(owner[size_t(nb[0] + long(nx) * (nb[1] + long(ny) * nb[2]))]);
+        }
+      }
+      std::sort(d.begin(), d.end());
+      d.erase(std::unique(d.begin(), d.end()), d.end());
+      // Tickets follow the chunk order: every upwind chunk must come first
+      // (it does, chunks being laid out by level), or a wait could outlast
+      // the items that would end it.
+      if (!d.empty() && (d.front() < 0 || d.back() >= c))
+        fail(OU_ERR_ARG, "sweep schedule: an upwind chunk does not precede its dependant");
+    }
+  }
+  PersistTables t;
+  t.dep_off.assign(chunks.size() + 1, 0);
+  for (size_t c = 0; c < chunks.size(); ++c) {
+    t.dep_off[c + 1] = t.dep_off[c] + int(deps[c].size());
+    t.dep_list.insert(t.dep_list.end(), deps[c].begin(), deps[c].end());
+  }
+  return t;
+}
+
+void check_sweep_watchdog(int *d_error, hipStream_t s) {
+  if (!d_error) return;
+  OU_HIP(hipStreamSynchronize(s));
+  int e = 0;
+  OU_HIP(hipMemcpy(&e, d_error, sizeof(int), hipMemcpyDeviceToHost));
+  if (e != 0) {
+    OU_HIP(hipMemset(d_error, 0, sizeof(int)));
+    fail(OU_ERR_TIMEOUT, "a wait of the persistent sweep for its upwind level gave up (the "
+                         "sweep's wait limit was reached); its results are invalid");
+  }
+}
+
+unsigned wait_polls(double seconds) {
+  // A poll takes at least the sleep (64 clocks) and one memory round trip,
+  // about half a microsecond: the poll bound ends the wait even if the wall
+  // clock were read at a wrong rate.
+  const double polls = std::max(0.0, seconds) * 2.0e6;
+  return polls >= 4.0e9 ? 4000000000u : unsigned(polls);
+}
+
+PersistView Plan::persist_view() const {
+  PersistView pv;
+  pv.dep_off = d_dep_off;
+  pv.dep_list = d_dep_list;
+  pv.done = d_done;
+  pv.ticket = d_done + size_t(n_chunks) * kMaxDirGroups;
+  pv.error = d_error;
+  pv.groups = dir_groups;
+  pv.n_items = int(n_chunks * dir_groups);
+  pv.wait_ticks = (unsigned long long)(std::max(0.0, wait_limit_s) * wall_clock_khz * 1000.0);
+  pv.max_polls = wait_polls(wait_limit_s);
+  return pv;
+}
+
 Plan::~Plan() {
   void *bufs[] = {d_ainv_mfma64, d_ainv_mfma, d_ainv, d_S, d_S0, d_slot_m2d, d_slot_c,
                   d_slot_d2m,  d_partial,  d_trace[0], d_trace[1], d_trace[2], d_oct_dirs,
                   d_oct_off,   d_chunks,   d_cells,    d_mat,      d_mom_l,    d_sigma_s,
                   d_source,    d_mass,     d_rowsum,   d_inflow,   d_x,        d_y,
-                  d_slot_cls, d_sigma_other};
+                  d_slot_cls, d_sigma_other, d_dep_off, d_dep_list, d_done, d_error};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   for (hipEvent_t e : ev)
@@ -197,7 +281,8 @@ Plan *create_plan(const ou_problem_desc *desc) {
     if (!sweep_supported(p.dim, p.p, p.mm, p.precision, p.waves, p.max_lds))
       fail(OU_ERR_UNSUPPORTED,
            "no GPU sweep kernel for this (dim, fe_order, n_mom): supported are "
-           "2D Q1-Q3 and 3D Q1-Q2 with up to 16 moments");
+           "2D Q1-Q3 and 3D Q1-Q2 with up to 16 moments, 2D Q4-Q5 with up to 10 "
+           "(L <= 3) and 3D Q3 with up to 4 (L <= 1)");
 
     // ---- directions per octant -------------------------------------------
     p.n_oct = p.dim == 2 ? 4 : 8;
@@ -347,6 +432,8 @@ Plan *create_plan(const ou_problem_desc *desc) {
     }
     p.level_chunk_off[p.n_levels] = int(chunks.size());
     p.n_chunks = long(chunks.size());
+    // upwind chunks of every chunk (the persistent sweep's dependencies)
+    p.deps = chunk_dependencies(chunks, cells, p.n, p.dim);
 
     // ---- direction groups ----------------------------------------------------
     // Enough workgroups per level launch to give each CU about two (a 2D level
@@ -354,15 +441,27 @@ Plan *create_plan(const ou_problem_desc *desc) {
     // directions per workgroup: each group re-reads its chunk's source and
     // writes its own partial moments.
     {
-      int dev = 0, cus = 256;
-      if (hipGetDevice(&dev) == hipSuccess)
+      int dev = 0, cus = 256, khz = 0;
+      if (hipGetDevice(&dev) == hipSuccess) {
         (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) == hipSuccess &&
+            khz > 0)
+          p.wall_clock_khz = double(khz);
+      }
+      p.cus = std::max(cus, 1);
       int max_dirs = 1;
       for (int o = 0; o < p.n_oct; ++o) max_dirs = std::max(max_dirs, p.oct_off[o + 1] - p.oct_off[o]);
       const double per_level = double(std::max<long>(p.n_chunks, 1)) / std::max(p.n_levels, 1);
       const int g_fill = int(std::ceil(2.0 * std::max(cus, 1) / per_level));
       const int g_dirs = std::max(1, max_dirs / std::max(p.waves, 1));
-      p.dir_groups = std::max(1, std::min({g_fill, g_dirs, kMaxDirGroups}));
+      // A sweep workgroup stages the tables of its direction group's range in
+      // LDS (at most kMaxSlotsPerGroup directions): octants with more
+      // directions need several groups.
+      p.min_dir_groups = (max_dirs + kMaxSlotsPerGroup - 1) / kMaxSlotsPerGroup;
+      if (p.min_dir_groups > kMaxDirGroups)
+        fail(OU_ERR_UNSUPPORTED, "more than " + std::to_string(kMaxSlotsPerGroup * kMaxDirGroups) +
+                                     " directions in one octant");
+      p.dir_groups = std::max(p.min_dir_groups, std::min({g_fill, g_dirs, kMaxDirGroups}));
       p.partial_groups = p.dir_groups;
     }
 
@@ -391,9 +490,6 @@ Plan *create_plan(const ou_problem_desc *desc) {
     // The face mass of axis a is (area_a) m1 (x) m1 in 3D and h_b m1 in 2D
     // (area_a = product of the cell sizes along the other axes); the kernel
     // applies the m1 factors and takes |Omega_a| area_a from this table.
-    for (int o = 0; o < p.n_oct; ++o)
-      if (p.oct_off[o + 1] - p.oct_off[o] > 64)
-        fail(OU_ERR_UNSUPPORTED, "more than 64 directions in one octant");
     std::vector<double> slot_m2d(size_t(p.n_dir) * p.mm, 0.0), slot_d2m(size_t(p.n_dir) * p.mm, 0.0),
         slot_c(size_t(p.n_dir) * 4, 0.0);
     for (int g = 0; g < p.n_dir; ++g) {
@@ -465,6 +561,17 @@ Plan *create_plan(const ou_problem_desc *desc) {
     upload(p.d_rowsum, rowsum);
     p.d_inflow = static_cast<double *>(dev_alloc(inflow.size() * 8));
     upload(p.d_inflow, inflow);
+    p.d_dep_off = static_cast<int *>(dev_alloc(p.deps.dep_off.size() * 4));
+    upload(p.d_dep_off, p.deps.dep_off);
+    p.d_dep_list = static_cast<int *>(dev_alloc(p.deps.dep_list.size() * 4));
+    upload(p.d_dep_list, p.deps.dep_list);
+    {
+      const size_t nc = size_t(p.n_chunks) * kMaxDirGroups + 1;
+      p.d_done = static_cast<unsigned *>(dev_alloc(nc * 4));
+      OU_HIP(hipMemset(p.d_done, 0, nc * 4));
+      p.d_error = static_cast<int *>(dev_alloc(4));
+      OU_HIP(hipMemset(p.d_error, 0, 4));
+    }
 
     launch_build_inverses(p);
     return pl;
@@ -477,6 +584,10 @@ Plan *create_plan(const ou_problem_desc *desc) {
 void set_sweep_shape(Plan &pl, int waves, int dir_groups) {
   if (dir_groups < 1 || dir_groups > kMaxDirGroups)
     fail(OU_ERR_ARG, "direction groups per octant must be in [1, 8]");
+  if (dir_groups < pl.min_dir_groups)
+    fail(OU_ERR_ARG, "this plan's octants hold more than " +
+                         std::to_string(kMaxSlotsPerGroup) + " directions: at least " +
+                         std::to_string(pl.min_dir_groups) + " direction groups are needed");
   if (!sweep_supported(pl.dim, pl.p, pl.mm, pl.precision, waves, pl.max_lds))
     fail(OU_ERR_UNSUPPORTED, "no sweep kernel with this many waves for this element");
   if (dir_groups > pl.partial_groups) {

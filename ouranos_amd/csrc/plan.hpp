@@ -11,6 +11,11 @@
 namespace ou {
 
 constexpr int kMaxDirGroups = 8;
+// (p+1)^2 entries of the 1D reference mass matrix, up to order 5.
+constexpr int kMaxP1Sq = 36;
+// Directions of one direction group whose tables a sweep workgroup stages in
+// LDS: an octant with more directions needs several groups.
+constexpr int kMaxSlotsPerGroup = 64;
 
 // One chunk: up to 64 cells of one sweep level of one octant.  Lanes 0-31
 // (count_lo cells, material `material`) and lanes 32-63 (count_hi cells,
@@ -48,8 +53,8 @@ struct SweepView {
   const void *slot_m2d;    // [g][MM] M2D row (sweep precision)
   const double *slot_d2m;  // [g][MM] D2M column
   const void *slot_c;      // [g][4] |Omega_a| x (area of the face normal to a) (sweep precision)
-  float m1f[16];           // 1D reference mass matrix on [0,1]
-  double m1d[16];
+  float m1f[kMaxP1Sq];     // 1D reference mass matrix on [0,1], (p+1) x (p+1)
+  double m1d[kMaxP1Sq];
   // Per axis: [dir][F][slot].  Slot of a cell in the octant's canonical
   // coordinates (ii,jj,kk): x -> jj + ny kk, y -> ii + nx kk, z -> (ii+jj) ny + jj
   // (skewed so that the cells of one level with equal kk and consecutive jj
@@ -60,6 +65,30 @@ struct SweepView {
   const Chunk *chunks;
   const int *cells;
   int dir_groups;          // workgroups per (chunk, octant), each a range of directions
+};
+
+// The persistent sweep's launch state (one launch sweeps every level).
+// Item k = (chunk k / groups, direction group k % groups), in the chunks'
+// level order.  The item of chunk c and group g waits until every upwind
+// chunk of c (dep_list[dep_off[c] .. dep_off[c+1]): the chunks of the
+// previous level of its octant that hold a neighbour one step upwind of one
+// of its cells) has set its flag done[chunk * groups + g], then sets its own
+// once its traces are stored.
+struct PersistView {
+  const int *dep_off;   // [chunk + 1]
+  const int *dep_list;
+  unsigned *done;       // [chunks * groups], cleared before each launch
+  unsigned *ticket;     // next item, cleared before each launch
+  int *error;           // sticky: 1 once a wait gave up (cleared by the host)
+  int n_items;
+  int groups;
+  unsigned long long wait_ticks;  // a wait gives up after this many wall-clock ticks
+  unsigned max_polls;             // ... or after this many polls
+};
+
+// Host-side dependency lists of a persistent schedule (CSR over chunks).
+struct PersistTables {
+  std::vector<int> dep_off, dep_list;
 };
 
 struct Plan {
@@ -80,6 +109,7 @@ struct Plan {
   // schedule
   int n_oct = 4, n_levels = 0, waves = 4, mm = 1;
   int dir_groups = 1;        // direction groups per octant (one workgroup each)
+  int min_dir_groups = 1;    // fewest groups that keep a group within kMaxSlotsPerGroup
   int partial_groups = 1;    // groups d_partial is allocated for
   std::vector<int> oct_dirs, oct_off, dir_oct;
   int n_cls = 0;
@@ -97,6 +127,17 @@ struct Plan {
   }
   long n_chunks = 0;
 
+  // How a sweep is launched: OU_SWEEP_LEVELS (one launch per level) or
+  // OU_SWEEP_PERSISTENT (one persistent launch).
+  int sweep_mode = OU_SWEEP_LEVELS;
+  PersistTables deps;                  // upwind chunks of every chunk (host)
+  int *d_dep_off = nullptr, *d_dep_list = nullptr;
+  unsigned *d_done = nullptr;          // [n_chunks * kMaxDirGroups + 1]: flags, ticket
+  int *d_error = nullptr;              // the persistent sweep's sticky error word
+  double wait_limit_s = 10.0;          // a persistent-sweep wait gives up after this long
+  double wall_clock_khz = 100000.0;    // rate of the device's wall clock
+  int cus = 256;
+
   // device buffers
   hipStream_t stream = nullptr;
   // the other octant parts' launches, forked from and joined back into the
@@ -113,7 +154,7 @@ struct Plan {
   int max_lds = 65536;
   void *d_slot_m2d = nullptr, *d_slot_c = nullptr;
   double *d_slot_d2m = nullptr, *d_partial = nullptr;
-  double m1[16] = {};
+  double m1[kMaxP1Sq] = {};
   void *d_trace[3] = {nullptr, nullptr, nullptr};
   long plane_size[3] = {0, 0, 0};
   int *d_oct_dirs = nullptr, *d_oct_off = nullptr;
@@ -129,6 +170,7 @@ struct Plan {
   ~Plan();
   size_t real_size() const { return precision == OU_FP32 ? 4 : 8; }
   SweepView view() const;
+  PersistView persist_view() const;
 };
 
 // One member of a batch, as the batched element-wise kernels see it.
@@ -170,9 +212,23 @@ struct BatchLaunch {
   const SweepView *views_src = nullptr;  // S0 = member source (sweep with source)
   const MemberView *members = nullptr;
   long max_elems = 0;                    // largest element count of any stage / member
+  // one persistent launch for every member's levels instead of level launches
+  int sweep_mode = OU_SWEEP_LEVELS;
+  PersistView persist = {};              // groups and n_items set by the launch
+  int cus = 256;
 };
 
 Plan *create_plan(const ou_problem_desc *desc);
+// Upwind chunks of every chunk of one plan (chunks laid out by level, their
+// cells in `cells`): for each cell, the chunk of the same octant that holds
+// its neighbour one step upwind along each axis.
+PersistTables chunk_dependencies(const std::vector<Chunk> &chunks, const std::vector<int> &cells,
+                                 const int n[3], int dim);
+// Sticky error word of the persistent sweep: if a wait gave up since the last
+// check, clears it and fails with OU_ERR_TIMEOUT (synchronises `s` first).
+void check_sweep_watchdog(int *d_error, hipStream_t s);
+// Poll bound of a persistent-sweep wait of `seconds`.
+unsigned wait_polls(double seconds);
 bool sweep_supported(int dim, int p, int mm, int precision, int waves, int max_lds);
 // Sets the sweep workgroup shape (waves per workgroup, direction groups per
 // octant).  Growing the group count reallocates the partial moments after a

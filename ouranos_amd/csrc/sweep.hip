@@ -14,6 +14,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <type_traits>
 
 #include "fe.hpp"
@@ -277,22 +278,37 @@ __global__ __launch_bounds__(64) void k_ainv_to_mfma64(const double *__restrict_
 // y = x - sum_o partial_o (Apply, mode 1), y += sum_o partial_o (sweep, mode
 // 0) or y = 0 + sum_o partial_o (a sweep into a fresh vector, mode 2), summed
 // over the n_parts = octants x direction groups partial moment vectors.
+// A persistent sweep whose watchdog gave up (error word set) leaves its
+// partial moments incomplete: the output is NaN then, so that no caller
+// mistakes it for a result.
 __device__ __forceinline__ void epilogue_one(long gid, int n_parts, long stride,
                                              const double *__restrict__ partial,
                                              const double *__restrict__ x,
-                                             double *__restrict__ y, int apply) {
+                                             double *__restrict__ y, int apply,
+                                             const int *error = nullptr) {
   double s = 0.0;
   for (int o = 0; o < n_parts; ++o) s += partial[o * stride + gid];
+  if (error && *error) s = __builtin_nan("");
   y[gid] = apply == 1 ? x[gid] - s : (apply == 2 ? 0.0 + s : y[gid] + s);
 }
 
 __global__ __launch_bounds__(256) void k_epilogue(long n, int n_parts, long stride,
                                                   const double *__restrict__ partial,
                                                   const double *__restrict__ x,
-                                                  double *__restrict__ y, int apply) {
+                                                  double *__restrict__ y, int apply,
+                                                  const int *error) {
   const long gid = long(blockIdx.x) * blockDim.x + threadIdx.x;
   if (gid >= n) return;
-  epilogue_one(gid, n_parts, stride, partial, x, y, apply);
+  epilogue_one(gid, n_parts, stride, partial, x, y, apply, error);
+}
+
+// Clears the persistent sweep's completion flags and ticket before its launch
+// (a kernel, so that captured graphs hold kernel nodes only).
+__global__ __launch_bounds__(256) void k_persist_reset(unsigned *done, long n,
+                                                       unsigned *ticket) {
+  const long gid = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (gid < n) done[gid] = 0u;
+  if (gid == 0) *ticket = 0u;
 }
 
 // ------------------------------------------------------------------------
@@ -350,7 +366,7 @@ __global__ __launch_bounds__(256) void k_batch_epilogue(const MemberView *__rest
                                                         int n_mem, const int *__restrict__ active,
                                                         const double *const *__restrict__ xs,
                                                         double *const *__restrict__ ys,
-                                                        int apply) {
+                                                        int apply, const int *error) {
   const long stride = long(gridDim.x) * blockDim.x;
   for (int u = 0; u < n_mem; ++u) {
     if (!active[u]) continue;
@@ -359,7 +375,7 @@ __global__ __launch_bounds__(256) void k_batch_epilogue(const MemberView *__rest
     double *y = ys[u];
     for (long gid = long(blockIdx.x) * blockDim.x + threadIdx.x; gid < m.vec_size;
          gid += stride)
-      epilogue_one(gid, m.n_parts, m.vec_size, m.partial, x, y, apply);
+      epilogue_one(gid, m.n_parts, m.vec_size, m.partial, x, y, apply, error);
   }
 }
 
@@ -428,6 +444,8 @@ inline unsigned grid_for(long n, int block) { return unsigned((n + block - 1) / 
 struct LevelKernels {
   void (*single)(const Plan &, const SweepView &, hipStream_t, hipEvent_t, hipEvent_t);
   void (*batch)(const BatchLaunch &, const SweepView *, hipStream_t, bool launch);
+  void (*persistent)(const Plan &, const SweepView &, hipStream_t, hipEvent_t, hipEvent_t);
+  void (*batch_persistent)(const BatchLaunch &, const SweepView *, hipStream_t, bool launch);
 };
 
 template <typename T, int DIM, int P, int MM, int W>
@@ -508,20 +526,85 @@ void launch_batch_levels_impl(const BatchLaunch &bl, const SweepView *views, hip
   OU_HIP(hipGetLastError());
 }
 
+// Resident workgroups of a persistent kernel on the whole device (its LDS
+// attribute must be set first).
+int resident_blocks(const void *kern, int threads, size_t lds, int cus) {
+  int per_cu = 0;
+  OU_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, threads, lds));
+  return std::max(1, per_cu) * std::max(1, cus);
+}
+
+void launch_persist_reset(const PersistView &pv, hipStream_t s) {
+  const long n = long(pv.n_items);  // chunks x groups flags
+  hipLaunchKernelGGL(k_persist_reset, dim3(std::max(1u, grid_for(n, 256))), dim3(256), 0, s,
+                     pv.done, n, pv.ticket);
+}
+
+template <typename T, int DIM, int P, int MM, int W>
+void launch_persistent_impl(const Plan &pl, const SweepView &v, hipStream_t s,
+                            hipEvent_t before, hipEvent_t after) {
+  using C = SweepCfg<T, DIM, P, MM, W>;
+  auto kern = k_sweep_persistent<T, DIM, P, MM, W>;
+  static int blocks_per_device = 0;
+  if (!blocks_per_device) {
+    set_lds_attr<T, DIM, P, MM, W>(reinterpret_cast<const void *>(kern));
+    blocks_per_device =
+        resident_blocks(reinterpret_cast<const void *>(kern), 64 * W, C::lds_bytes, pl.cus);
+  }
+  const PersistView pv = pl.persist_view();
+  if (long(pl.n_chunks) * pl.dir_groups > long(INT32_MAX))
+    fail(OU_ERR_UNSUPPORTED, "too many sweep items for one persistent launch");
+  if (before) OU_HIP(hipEventRecord(before, s));
+  launch_persist_reset(pv, s);
+  const int grid = std::max(1, std::min(blocks_per_device, pv.n_items));
+  if (pv.n_items > 0)
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * W), C::lds_bytes, s, v, pv);
+  if (after) OU_HIP(hipEventRecord(after, s));
+  OU_HIP(hipGetLastError());
+}
+
+template <typename T, int DIM, int P, int MM, int W>
+void launch_batch_persistent_impl(const BatchLaunch &bl, const SweepView *views, hipStream_t s,
+                                  bool launch) {
+  using C = SweepCfg<T, DIM, P, MM, W>;
+  auto kern = k_sweep_persistent_batch<T, DIM, P, MM, W>;
+  static int blocks_per_device = 0;
+  if (!launch) {
+    set_lds_attr<T, DIM, P, MM, W>(reinterpret_cast<const void *>(kern));
+    if (!blocks_per_device)
+      blocks_per_device =
+          resident_blocks(reinterpret_cast<const void *>(kern), 64 * W, C::lds_bytes, bl.cus);
+    return;
+  }
+  PersistView pv = bl.persist;
+  pv.groups = bl.dir_groups;
+  pv.n_items = int(bl.level_off.back() * long(bl.dir_groups));
+  launch_persist_reset(pv, s);
+  const int grid = std::max(1, std::min(blocks_per_device, pv.n_items));
+  if (pv.n_items > 0)
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * W), C::lds_bytes, s, views, bl.chunk_view,
+                       bl.chunks, bl.active, pv);
+  OU_HIP(hipGetLastError());
+}
+
 template <typename T, int DIM, int P, int MM, int W>
 const LevelKernels *kernels_of() {
   static const LevelKernels k{launch_levels_impl<T, DIM, P, MM, W>,
-                              launch_batch_levels_impl<T, DIM, P, MM, W>};
+                              launch_batch_levels_impl<T, DIM, P, MM, W>,
+                              launch_persistent_impl<T, DIM, P, MM, W>,
+                              launch_batch_persistent_impl<T, DIM, P, MM, W>};
   return &k;
 }
 
-template <typename T, int DIM, int P, int W>
+// MAX_MM bounds the moment counts built for an element: the higher orders
+// keep their FP64 accumulators (MM x N / W per lane) within the registers.
+template <typename T, int DIM, int P, int W, int MAX_MM = 16>
 const LevelKernels *pick_mm(int mm) {
   switch (mm) {
     case 1: return kernels_of<T, DIM, P, 1, W>();
-    case 4: return kernels_of<T, DIM, P, 4, W>();
-    case 10: if constexpr (W == 4) return kernels_of<T, DIM, P, 10, W>(); break;
-    case 16: if constexpr (W == 4) return kernels_of<T, DIM, P, 16, W>(); break;
+    case 4: if constexpr (MAX_MM >= 4) return kernels_of<T, DIM, P, 4, W>(); break;
+    case 10: if constexpr (W == 4 && MAX_MM >= 10) return kernels_of<T, DIM, P, 10, W>(); break;
+    case 16: if constexpr (W == 4 && MAX_MM >= 16) return kernels_of<T, DIM, P, 16, W>(); break;
   }
   return nullptr;
 }
@@ -534,8 +617,11 @@ const LevelKernels *pick(int dim, int p, int mm, int waves) {
     if (dim == 2 && p == 1) return pick_mm<T, 2, 1, 4>(mm);
     if (dim == 2 && p == 2) return pick_mm<T, 2, 2, 4>(mm);
     if (dim == 2 && p == 3) return pick_mm<T, 2, 3, 4>(mm);
+    if (dim == 2 && p == 4) return pick_mm<T, 2, 4, 4, 10>(mm);
+    if (dim == 2 && p == 5) return pick_mm<T, 2, 5, 4, 10>(mm);
     if (dim == 3 && p == 1) return pick_mm<T, 3, 1, 4>(mm);
     if (dim == 3 && p == 2) return pick_mm<T, 3, 2, 4>(mm);
+    if (dim == 3 && p == 3) return pick_mm<T, 3, 3, 4, 4>(mm);
   } else if (dim == 3 && mm <= 4) {
     if (p == 1) {
       if (waves == 2) return pick_mm<T, 3, 1, 2>(mm);
@@ -572,7 +658,7 @@ void scatter_impl(Plan &pl, const double *d_x, const double *d_sigma, bool accum
                        pl.L, pl.d_mom_l, pl.d_mass, S, int(accumulate));              \
     break;
   switch (pl.N) {
-    OU_SCAT(4) OU_SCAT(9) OU_SCAT(16) OU_SCAT(8) OU_SCAT(27)
+    OU_SCAT(4) OU_SCAT(9) OU_SCAT(16) OU_SCAT(25) OU_SCAT(36) OU_SCAT(8) OU_SCAT(27) OU_SCAT(64)
     default: fail(OU_ERR_UNSUPPORTED, "scatter source: unsupported element size");
   }
 #undef OU_SCAT
@@ -590,7 +676,7 @@ void source0_impl(Plan &pl, hipStream_t s) {
                        pl.d_rowsum, factor, static_cast<T *>(pl.d_S0));               \
     break;
   switch (pl.N) {
-    OU_SRC(4) OU_SRC(9) OU_SRC(16) OU_SRC(8) OU_SRC(27)
+    OU_SRC(4) OU_SRC(9) OU_SRC(16) OU_SRC(25) OU_SRC(36) OU_SRC(8) OU_SRC(27) OU_SRC(64)
     default: fail(OU_ERR_UNSUPPORTED, "source: unsupported element size");
   }
 #undef OU_SRC
@@ -631,7 +717,7 @@ SweepView Plan::view() const {
   v.slot_m2d = d_slot_m2d;
   v.slot_d2m = d_slot_d2m;
   v.slot_c = d_slot_c;
-  for (int k = 0; k < 16; ++k) {
+  for (int k = 0; k < kMaxP1Sq; ++k) {
     v.m1f[k] = float(m1[k]);
     v.m1d[k] = m1[k];
   }
@@ -738,14 +824,18 @@ void launch_sweep_levels(Plan &pl, const void *S0, hipStream_t s, hipEvent_t bef
   const LevelKernels *k = pl.precision == OU_FP32 ? pick<float>(pl.dim, pl.p, pl.mm, pl.waves)
                                                   : pick<double>(pl.dim, pl.p, pl.mm, pl.waves);
   if (!k) fail(OU_ERR_UNSUPPORTED, "no sweep kernel for this element / moment count");
-  k->single(pl, v, s, before, after);
+  if (pl.sweep_mode == OU_SWEEP_PERSISTENT)
+    k->persistent(pl, v, s, before, after);
+  else
+    k->single(pl, v, s, before, after);
 }
 
 void launch_epilogue(Plan &pl, const double *d_x, double *d_y, bool apply, hipStream_t s) {
   const long n = pl.vec_size;
   hipLaunchKernelGGL(k_epilogue, dim3(grid_for(n, 256)), dim3(256), 0, s, n,
                      pl.n_oct * pl.dir_groups, pl.vec_size, pl.d_partial, d_x, d_y,
-                     int(apply));
+                     int(apply),
+                     pl.sweep_mode == OU_SWEEP_PERSISTENT ? pl.d_error : nullptr);
   OU_HIP(hipGetLastError());
 }
 
@@ -778,7 +868,7 @@ void batch_scatter_impl(const BatchLaunch &b, const double *const *d_xs, hipStre
                        b.n_members, b.active, d_xs);                                   \
     break;
   switch (b.N) {
-    OU_BSCAT(4) OU_BSCAT(9) OU_BSCAT(16) OU_BSCAT(8) OU_BSCAT(27)
+    OU_BSCAT(4) OU_BSCAT(9) OU_BSCAT(16) OU_BSCAT(25) OU_BSCAT(36) OU_BSCAT(8) OU_BSCAT(27) OU_BSCAT(64)
     default: fail(OU_ERR_UNSUPPORTED, "batched scatter: unsupported element size");
   }
 #undef OU_BSCAT
@@ -793,7 +883,7 @@ void batch_source0_impl(const BatchLaunch &b, hipStream_t s) {
                        b.n_members, b.active);                                         \
     break;
   switch (b.N) {
-    OU_BSRC(4) OU_BSRC(9) OU_BSRC(16) OU_BSRC(8) OU_BSRC(27)
+    OU_BSRC(4) OU_BSRC(9) OU_BSRC(16) OU_BSRC(25) OU_BSRC(36) OU_BSRC(8) OU_BSRC(27) OU_BSRC(64)
     default: fail(OU_ERR_UNSUPPORTED, "batched source: unsupported element size");
   }
 #undef OU_BSRC
@@ -812,12 +902,26 @@ void batch_trace_init(const BatchLaunch &b, bool with_source, hipStream_t s) {
 void batch_epilogue(const BatchLaunch &b, const double *const *d_xs, double *const *d_ys,
                     int mode, hipStream_t s) {
   hipLaunchKernelGGL(k_batch_epilogue, dim3(batch_grid(b)), dim3(256), 0, s, b.members,
-                     b.n_members, b.active, d_xs, d_ys, mode);
+                     b.n_members, b.active, d_xs, d_ys, mode,
+                     b.sweep_mode == OU_SWEEP_PERSISTENT ? b.persist.error : nullptr);
+}
+
+// The merged levels of every member: one persistent launch or level launches.
+void batch_levels(const BatchLaunch &b, const LevelKernels *k, const SweepView *views,
+                  hipStream_t s) {
+  if (b.sweep_mode == OU_SWEEP_PERSISTENT)
+    k->batch_persistent(b, views, s, true);
+  else
+    k->batch(b, views, s, true);
 }
 
 }  // namespace
 
-void batch_prepare(const BatchLaunch &b) { batch_kernels(b)->batch(b, nullptr, nullptr, false); }
+void batch_prepare(const BatchLaunch &b) {
+  const LevelKernels *k = batch_kernels(b);
+  k->batch(b, nullptr, nullptr, false);
+  k->batch_persistent(b, nullptr, nullptr, false);
+}
 
 void batch_apply(const BatchLaunch &b, const double *const *d_xs, double *const *d_ys,
                  hipStream_t s) {
@@ -827,7 +931,7 @@ void batch_apply(const BatchLaunch &b, const double *const *d_xs, double *const 
   else
     batch_scatter_impl<double>(b, d_xs, s);
   batch_trace_init(b, false, s);
-  k->batch(b, b.views, s, true);
+  batch_levels(b, k, b.views, s);
   batch_epilogue(b, d_xs, d_ys, 1, s);
   OU_HIP(hipGetLastError());
 }
@@ -841,7 +945,7 @@ void batch_sweep(const BatchLaunch &b, double *const *d_ys, bool with_source, hi
       batch_source0_impl<double>(b, s);
   }
   batch_trace_init(b, with_source, s);
-  k->batch(b, with_source ? b.views_src : b.views, s, true);
+  batch_levels(b, k, with_source ? b.views_src : b.views, s);
   batch_epilogue(b, nullptr, d_ys, 0, s);
   OU_HIP(hipGetLastError());
 }
@@ -857,7 +961,7 @@ void batch_si_step(const BatchLaunch &b, const double *const *d_xs, double *cons
     batch_source0_impl<double>(b, s);
   }
   batch_trace_init(b, true, s);
-  k->batch(b, b.views_src, s, true);
+  batch_levels(b, k, b.views_src, s);
   batch_epilogue(b, nullptr, d_ys, 2, s);
   hipLaunchKernelGGL(k_batch_change_norms, dim3(unsigned(partial_blocks), unsigned(b.n_members)),
                      dim3(256), 0, s, b.members, b.active, d_xs,

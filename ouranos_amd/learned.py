@@ -233,6 +233,108 @@ def evaluate(net, n_instances, seed0=EVAL_SEED0, n=32, tol=1e-8, restart=30, max
     return rows
 
 
+def gcr(apply_a, b, precond, tol, restart, max_it, x_exact=None):
+    """Flexible GCR (Eisenstat, Elman and Schultz, SIAM J. Numer. Anal. 20
+    (1983)) restarted every `restart` steps: step k applies the preconditioner
+    to the current residual, z_k = M(r_k, x_k), makes A z_k orthonormal to the
+    previous images (classical Gram-Schmidt, two passes) and minimises the
+    residual over the search directions.  Unlike FGMRES the preconditioner
+    always sees the true residual, whose exact correction A^-1 r_k = x* - x_k
+    is known when the solution is: that is what the ideal preconditioner of
+    `ceiling` uses.  Returns (x, steps, relative residual history)."""
+    x = np.zeros_like(b)
+    r = b.copy()
+    bn = float(np.linalg.norm(b))
+    hist = [1.0]
+    ps, qs = [], []
+    k = 0
+    while k < max_it and hist[-1] >= tol:
+        z = precond(r, x)
+        w = apply_a(z)
+        k += 1
+        for _ in range(2):
+            for p, q in zip(ps, qs):
+                a = float(q @ w)
+                w -= a * q
+                z -= a * p
+        nw = float(np.linalg.norm(w))
+        if not nw > 0.0:
+            break
+        q, p = w / nw, z / nw
+        alpha = float(q @ r)
+        x += alpha * p
+        r -= alpha * q
+        ps.append(p)
+        qs.append(q)
+        if len(ps) == restart:
+            ps, qs = [], []
+            r = b - apply_a(x)  # restart from the true residual
+        hist.append(float(np.linalg.norm(r)) / bn)
+    return x, k, hist
+
+
+def ceiling(net, n_instances=3, seed0=EVAL_SEED0, n=32, grid=32, tol=1e-8, restart=30,
+            max_it=600, log=print):
+    """How far the network's correction space can go.  On held-out instances
+    of the thick family (FP64 operator), flexible GCR(restart) to `tol` with
+      plain    z = r;
+      learned  z = r + P s N(R r / s), the network as the FGMRES preconditioner
+               applies it;
+      ideal    z = r + P R (A^-1 r - r), with A^-1 r = x* - x exact (x* from
+               GMRES to 1e-12): the correction the network is trained to
+               predict, at its exact value, in the same coarse corner space;
+      exact    z = A^-1 r (a one-step solve; checks the driver).
+    Returns one record per instance."""
+    quad = ou.Quadrature(ou.LS, 3, sn=8)
+    rows = []
+    for i in range(n_instances):
+        spec = instance(seed0 + i, n=n, quad=quad)
+        op = spec.operator(ou.FP64)
+        rhs = _rhs_of(op)
+        x_star, info = op.solve_rhs(rhs, solver=ou.GMRES, tol=1e-12, max_it=4000)
+        _, mat = op.coarse_features(grid)
+
+        def coarse(v):
+            return op.coarse_features(grid, v)[0]
+
+        def prec_learned(r, x):
+            rk = coarse(r)
+            f, s_r = features_of(rk, mat)
+            if not s_r > 0:
+                return r.copy()
+            pred = net.predict(f[None])[0]
+            return op.coarse_prolong(grid, pred * s_r, z=r)
+
+        def prec_ideal(r, x):
+            e = x_star - x
+            return op.coarse_prolong(grid, (coarse(e) - coarse(r)).astype(np.float32), z=r)
+
+        def prec_exact(r, x):
+            return x_star - x
+
+        row = {"seed": seed0 + i, "reference_gmres_iterations": info["iterations"]}
+        for name, m in (("plain", lambda r, x: r.copy()), ("learned", prec_learned),
+                        ("ideal", prec_ideal), ("exact", prec_exact)):
+            t0 = time.perf_counter()
+            x, k, hist = gcr(op.Apply, rhs, m, tol, restart, max_it)
+            diff = float(np.linalg.norm(x[:spec.n_dofs] - x_star[:spec.n_dofs]) /
+                         np.linalg.norm(x_star[:spec.n_dofs]))
+            row[name] = {"steps": k, "rel_residual": hist[-1], "scalar_flux_diff": diff,
+                         "seconds": time.perf_counter() - t0}
+            log(f"  instance {seed0 + i}: GCR({restart}) {name}: {k} steps, residual "
+                f"{hist[-1]:.2e}, scalar-flux difference {diff:.1e}")
+        op.close()
+        rows.append(row)
+    return rows
+
+
+def _rhs_of(op):
+    """The instance's right-hand side D L^-1 (q + inflow) (ouranos.cc:149-158):
+    the sweep of the source with a zero scattering source."""
+    op.compute_scattering_source(np.zeros(op.n))
+    return op.sweep(None, with_source=True)
+
+
 def save(net, path):
     """Parameters (.npy, float32) plus a JSON sidecar with grid and mode."""
     np.save(path, net.get_params())
@@ -291,6 +393,11 @@ def main(argv=None):
     t.add_argument("--pair-restart", type=int, default=FGMRES_RESTART,
                    help="restart length of the on-policy FGMRES pairs")
     t.add_argument("--eval-instances", type=int, default=0)
+    c = sub.add_parser("ceiling")
+    c.add_argument("--weights", required=True)
+    c.add_argument("--instances", type=int, default=3)
+    c.add_argument("--restart", type=int, default=30)
+    c.add_argument("--out", default=None)
     e = sub.add_parser("evaluate")
     e.add_argument("--weights", required=True)
     e.add_argument("--instances", type=int, default=200)
@@ -348,6 +455,17 @@ def main(argv=None):
         with open(os.path.splitext(a.out)[0] + "_report.json", "w") as f:
             json.dump(rep, f, indent=1)
         log("[learned] done")
+    elif a.cmd == "ceiling":
+        net = load(a.weights)
+        rows = ceiling(net, a.instances, restart=a.restart, log=log)
+        summary = {m: float(np.mean([r[m]["steps"] for r in rows]))
+                   for m in ("plain", "learned", "ideal", "exact")}
+        log(json.dumps({"mean_steps": summary}))
+        if a.out:
+            with open(a.out, "w") as f:
+                json.dump({"mean_steps": summary, "rows": rows,
+                           "command": "python -m ouranos_amd.learned " +
+                                      " ".join(argv or sys.argv[1:])}, f, indent=1)
     else:
         net = load(a.weights)
         rows = evaluate(net, a.instances, log=log)

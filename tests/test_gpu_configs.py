@@ -102,11 +102,13 @@ def test_c1_uniform_2d_apply_and_source_iteration(oracle):
     # the same fixed point by GMRES, and the FP32 sweep's error stated
     phi_g, info_g = op.solve(solver=ou.GMRES, tol=1e-12, max_it=500)
     assert info_g["converged"] and rel_err(phi_g, phi_ref) < 1e-9
+    # FP32 sweeps: the change of source iteration stalls at the FP32 rounding
+    # level (~1e-7), so the FP32 solve is run to 1e-6
     op32 = spec.operator(ou.FP32)
     err32 = rel_err(x - op32.Apply(x), x - pb.apply(x))
-    phi32, info32 = op32.solve(solver=ou.SI, tol=1e-10, max_it=1000)
+    phi32, info32 = op32.solve(solver=ou.SI, tol=1e-6, max_it=1000)
     err_phi32 = rel_err(phi32, phi_ref)
-    print(f"[gpu] C1 FP32: sweep part error {err32:.2e}; SI to 1e-10 in "
+    print(f"[gpu] C1 FP32: sweep part error {err32:.2e}; SI to 1e-6 in "
           f"{info32['iterations']} iterations, scalar flux error {err_phi32:.2e}", flush=True)
     assert err32 < FP32_TOL and err_phi32 < FP32_TOL
 

@@ -28,6 +28,13 @@ def small_cases():
         "headline_3d_q2_L1": lambda: P.headline_3d(n=5, block=2, n_polar=4, n_azim=8, L=1),
         "headline_3d_q2_L2": lambda: P.headline_3d(n=4, block=2, n_polar=4, n_azim=8, L=2),
         "headline_3d_q1_L3": lambda: P.headline_3d(n=5, block=2, n_polar=4, n_azim=8, L=3, p=1),
+        # the reference's higher element orders (RadiativeTransfer.cc:1108-1117)
+        "hetero_2d_q4_L3": lambda: P.hetero_2d(n=6, block=3, n_polar=4, n_azim=8, L=3, p=4),
+        "hetero_2d_q5_L2": lambda: P.hetero_2d(n=5, block=2, n_polar=4, n_azim=8, L=2, p=5),
+        "headline_3d_q3_L1": lambda: P.headline_3d(n=4, block=2, n_polar=4, n_azim=8, L=1, p=3),
+        # 128 directions per octant: two direction groups of 64 at least
+        "batch_3d_q1_glc16x64": lambda: P.batch_3d(
+            n=4, block=2, quad=ou.Quadrature(ou.PRODUCT_GLC, 3, n_polar=16, n_azim=64, L_max=0)),
     }
 
 
@@ -156,10 +163,15 @@ def test_operator_surface_and_errors():
         op.set_group(1)
     assert e.value.code == -4
     op.set_group(0)
-    # 3D Q3 has no GPU sweep kernel: reported, not silently run elsewhere
+    # 3D Q4 and 3D Q3 beyond L = 1 have no GPU sweep kernel: reported, not
+    # silently run elsewhere
     q = ou.Quadrature(ou.LS, 3, sn=4)
     with pytest.raises(ou.OuError) as e:
-        ou.RadiativeTransfer(3, 3, (2, 2, 2), (1, 1, 1), q, [1.0], [[0.5]], np.zeros(8))
+        ou.RadiativeTransfer(3, 4, (2, 2, 2), (1, 1, 1), q, [1.0], [[0.5]], np.zeros(8))
+    assert e.value.code == -4
+    q3 = ou.Quadrature(ou.LS, 3, sn=4, L_max=2)
+    with pytest.raises(ou.OuError) as e:
+        ou.RadiativeTransfer(3, 3, (2, 2, 2), (1, 1, 1), q3, [1.0], [[0.5, 0.1, 0.0]], np.zeros(8))
     assert e.value.code == -4
     # non-finite cross section rejected at plan creation
     with pytest.raises(ou.OuError) as e:
@@ -327,3 +339,25 @@ def test_device_nonfinite_ends_the_solve(solver):
         op.solve_rhs(rhs, solver=solver, tol=1e-10, max_it=100000)
     assert e.value.code == -5, e.value
     assert b"non-finite" in ou.lib().ou_last_error()
+
+
+def test_more_than_64_directions_per_octant_need_groups(oracle):
+    """An octant's directions are staged in LDS 64 at a time, one direction
+    group per workgroup: 128 per octant need at least two groups.  The plan
+    starts with enough; fewer are refused; every allowed shape matches the
+    oracle."""
+    q = ou.Quadrature(ou.PRODUCT_GLC, 3, n_polar=16, n_azim=64, L_max=1)
+    assert q.n_dir // 8 == 128
+    spec = P.headline_3d(n=3, block=2, quad=q, p=1)
+    op = spec.operator(ou.FP64)
+    w, g = op.sweep_shape
+    assert g >= 2
+    with pytest.raises(ou.OuError) as e:
+        op.sweep_shape = (w, 1)
+    assert e.value.code == -1
+    x = P.seeded_vector(spec.size, 8)
+    ref = oracle_problem(oracle, spec).apply(x)
+    for shape in [(4, 2), (4, 3), (2, 8)]:
+        op.sweep_shape = shape
+        assert rel_err(op.Apply(x), ref) < FP64_TOL, shape
+    op.close()

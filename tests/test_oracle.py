@@ -161,13 +161,16 @@ def test_lu_crout_matches_dense_solve(oracle):
             assert np.abs(L @ U - A[P]).max() < 1e-10 * np.abs(A).max() * n
 
 
-@pytest.mark.parametrize("dim,p", [(2, 1), (2, 2), (2, 3), (3, 1), (3, 2)])
+@pytest.mark.parametrize("dim,p", [(2, 1), (2, 2), (2, 3), (2, 4), (2, 5), (3, 1), (3, 2), (3, 3)])
 def test_fe_closed_forms(oracle, dim, p):
     h = [0.5, 0.25, 2.0][:dim]
     M, G, E, U, nrm = oracle.fe_cell(dim, p, h)
     # row sums of M = integral of each basis function: tensor product of the
-    # 1D closed Newton-Cotes weights (trapezoid for Q1, Simpson for Q2, 3/8 for Q3)
-    nc = {1: [1 / 2, 1 / 2], 2: [1 / 6, 2 / 3, 1 / 6], 3: [1 / 8, 3 / 8, 3 / 8, 1 / 8]}[p]
+    # 1D closed Newton-Cotes weights (trapezoid for Q1, Simpson for Q2, 3/8 for
+    # Q3, Boole for Q4, the six-point rule for Q5)
+    nc = {1: [1 / 2, 1 / 2], 2: [1 / 6, 2 / 3, 1 / 6], 3: [1 / 8, 3 / 8, 3 / 8, 1 / 8],
+          4: [7 / 90, 32 / 90, 12 / 90, 32 / 90, 7 / 90],
+          5: [19 / 288, 75 / 288, 50 / 288, 50 / 288, 75 / 288, 19 / 288]}[p]
     w = np.array([1.0])
     for a in range(dim):
         w = np.kron(np.array(nc) * h[a], w)
@@ -216,7 +219,9 @@ def _global_transport_solve(O, dim, p, n, h, om, M2D, D2M, sigt, S):
 
 
 @pytest.mark.parametrize("dim,p,n,L", [(2, 1, (3, 3), 0), (2, 2, (2, 3), 1),
-                                       (3, 1, (2, 2, 2), 1), (3, 2, (2, 1, 2), 0)])
+                                       (3, 1, (2, 2, 2), 1), (3, 2, (2, 1, 2), 0),
+                                       (2, 4, (2, 2), 1), (2, 5, (2, 1), 0),
+                                       (3, 3, (2, 1, 2), 1)])
 def test_sweep_equals_global_dense_solve(oracle, dim, p, n, L):
     """One sweep = exact inverse of the block-lower-triangular transport
     operator: compare with a dense solve of the global system."""
