@@ -4,7 +4,7 @@ One operator application of a named workload on device buffers, followed by
 a 1 GiB device copy through the library's plain copy kernel (the reference
 point that scales the FETCH_SIZE / WRITE_SIZE traffic counters).
 
-    python -m ouranos_amd.profile_driver [workload] [fp32|fp64]
+    python -m ouranos_amd.profile_driver [workload] [fp32|fp64] [levels|persistent]
 """
 import ctypes
 import sys
@@ -18,6 +18,8 @@ def main(argv):
     prec = ou.FP64 if len(argv) > 2 and argv[2] == "fp64" else ou.FP32
     spec = getattr(P, name)()
     op = spec.operator(prec)
+    if len(argv) > 3:
+        op.sweep_mode = ou.SWEEP_PERSISTENT if argv[3] == "persistent" else ou.SWEEP_LEVELS
     x = P.seeded_vector(spec.size, 1)
     d_x = ou.DeviceBuffer(x.nbytes)
     d_y = ou.DeviceBuffer(x.nbytes)
