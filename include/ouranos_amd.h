@@ -35,7 +35,7 @@ typedef enum {
 
 typedef enum { OU_FP64 = 0, OU_FP32 = 1 } ou_precision;
 /* How a sweep is launched (ou_set_sweep_mode). */
-typedef enum { OU_SWEEP_LEVELS = 0, OU_SWEEP_PERSISTENT = 1 } ou_sweep_mode_kind;
+typedef enum { OU_SWEEP_LEVELS = 0, OU_SWEEP_PERSISTENT = 1, OU_SWEEP_TILES = 2 } ou_sweep_mode_kind;
 /* OU_BC_REFLECTIVE is what a parameter file may name (Parameters.cc:59-70);
  * the reference never implements it in the sweep, and plans reject it with
  * OU_ERR_UNSUPPORTED. */
@@ -345,12 +345,18 @@ int ou_sweep_streams(const ou_plan *plan, int *n_streams);
 /* How a sweep is launched.  OU_SWEEP_LEVELS: one launch per wavefront level
  * (octant parts per ou_set_sweep_streams).  OU_SWEEP_PERSISTENT: one launch
  * for the whole sweep; resident workgroups take (chunk, direction group)
- * items in level order and start one as soon as the upwind level of its own
- * octant and group is done, the dataflow of the reference's scheduler
- * (RadiativeTransfer.cc:605-778, Task.cc:52-62) with the level as the unit
- * of readiness.  Both modes run the same per-chunk arithmetic: results are
- * bit-for-bit the same.  A new plan picks the mode its workload runs faster
- * in. */
+ * items in level order and start one as soon as the chunks upwind of it in
+ * its own octant and group are done, the dataflow of the reference's
+ * scheduler (RadiativeTransfer.cc:605-778, Task.cc:52-62) with the chunk as
+ * the unit of readiness.  OU_SWEEP_TILES (2D, Q1-Q2, one material per 8x8
+ * tile of cells): one launch per tile level; a workgroup sweeps a tile's 15
+ * cell levels with the traces in registers (OU_ERR_UNSUPPORTED for other
+ * plans).  All modes run the same per-cell arithmetic in the same order: at
+ * equal direction groups the results are bit-for-bit the same.
+ * ou_set_sweep_shape / ou_sweep_shape set and read the shape of the mode in
+ * use (for OU_SWEEP_TILES: waves of 8 directions each, 1, 2 or 4, and
+ * direction groups per quadrant).  A new plan starts in OU_SWEEP_TILES where
+ * it has one and in OU_SWEEP_LEVELS otherwise. */
 int ou_set_sweep_mode(ou_plan *plan, int mode);
 int ou_sweep_mode(const ou_plan *plan, int *mode);
 

@@ -30,7 +30,7 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libouranos_
 FP64, FP32 = 0, 1
 LS, GLC, PRODUCT_GLC = 0, 1, 2
 SI, GMRES, BICGSTAB, FGMRES = 0, 1, 2, 3
-SWEEP_LEVELS, SWEEP_PERSISTENT = 0, 1
+SWEEP_LEVELS, SWEEP_PERSISTENT, SWEEP_TILES = 0, 1, 2
 BC_VACUUM, BC_ISOTROPIC, BC_MOST_NORMAL = 0, 1, 2
 
 STATUS = {0: "OK", -1: "ARG", -2: "SIZE", -3: "HIP", -4: "UNSUPPORTED", -5: "NONFINITE",
@@ -532,8 +532,10 @@ class RadiativeTransfer:
 
     @property
     def sweep_mode(self):
-        """SWEEP_LEVELS (one launch per wavefront level) or SWEEP_PERSISTENT
-        (one persistent launch per sweep)."""
+        """SWEEP_LEVELS (one launch per wavefront level), SWEEP_PERSISTENT
+        (one persistent launch per sweep) or SWEEP_TILES (2D Q1-Q2 with one
+        material per 8x8 tile: one launch per tile level).  sweep_shape reads
+        and sets the shape of the mode in use."""
         m = ctypes.c_int()
         _check(lib().ou_sweep_mode(self._plan, ctypes.byref(m)))
         return m.value

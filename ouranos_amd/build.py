@@ -16,7 +16,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
 SOURCES = ["quadrature.cpp", "fe.cpp", "plan.cpp", "solvers.cpp", "capi.cpp",
-           "sweep.hip", "vec.hip", "unet.hip", "learned.hip", "batch.cpp", "inp.cpp", "output.cpp"]
+           "sweep.hip", "sweep_tile.hip", "vec.hip", "unet.hip", "learned.hip", "batch.cpp", "inp.cpp", "output.cpp"]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function",
           "-Wno-unused-result", f"--offload-arch={ARCH}",

@@ -272,6 +272,8 @@ void Batch::build_views() {
     m.L = pl.L;
     m.n_dir = pl.n_dir;
     m.n_parts = pl.n_oct * pl.dir_groups;
+    m.groups = pl.dir_groups;
+    m.oct_mask = pl.oct_mask;
     m.mat = pl.d_mat;
     m.sigma_s = pl.d_sigma_s;
     m.mom_l = pl.d_mom_l;

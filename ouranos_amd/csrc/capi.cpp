@@ -562,6 +562,9 @@ int ou_time_apply(ou_plan *plan, const double *d_x, double *d_y, int warmup, int
     const int ns = pl.sweep_streams;
     if (pl.sweep_mode == OU_SWEEP_PERSISTENT)
       launches = 2;  // counter reset + the persistent sweep
+    else if (pl.sweep_mode == OU_SWEEP_TILES)
+      for (size_t l = 0; l + 1 < pl.tile_level_off.size(); ++l)
+        launches += pl.tile_level_off[l + 1] > pl.tile_level_off[l];
     else
       for (int l = 0; l < pl.n_levels; ++l)
         for (int k = 0; k < ns; ++k)
@@ -581,7 +584,12 @@ int ou_time_apply(ou_plan *plan, const double *d_x, double *d_y, int warmup, int
 
 int ou_set_sweep_shape(ou_plan *plan, int waves, int dir_groups) {
   return guarded([&] {
-    ou::set_sweep_shape(plan_of(plan), waves, dir_groups);
+    ou::Plan &pl = plan_of(plan);
+    OU_HIP(hipStreamSynchronize(pl.stream));
+    if (pl.sweep_mode == OU_SWEEP_TILES)
+      ou::set_tile_shape(pl, waves, dir_groups);
+    else
+      ou::set_sweep_shape(pl, waves, dir_groups);
     return int(OU_OK);
   });
 }
@@ -589,8 +597,9 @@ int ou_set_sweep_shape(ou_plan *plan, int waves, int dir_groups) {
 int ou_sweep_shape(const ou_plan *plan, int *waves, int *dir_groups) {
   return guarded([&] {
     const ou::Plan &pl = plan_of(const_cast<ou_plan *>(plan));
-    if (waves) *waves = pl.waves;
-    if (dir_groups) *dir_groups = pl.dir_groups;
+    const bool tiles = pl.sweep_mode == OU_SWEEP_TILES;
+    if (waves) *waves = tiles ? pl.tile_waves : pl.waves;
+    if (dir_groups) *dir_groups = tiles ? pl.tile_groups : pl.dir_groups;
     return int(OU_OK);
   });
 }
@@ -617,10 +626,8 @@ int ou_sweep_streams(const ou_plan *plan, int *n_streams) {
 int ou_set_sweep_mode(ou_plan *plan, int mode) {
   return guarded([&] {
     ou::Plan &pl = plan_of(plan);
-    if (mode != OU_SWEEP_LEVELS && mode != OU_SWEEP_PERSISTENT)
-      ou::fail(OU_ERR_ARG, "sweep mode must be OU_SWEEP_LEVELS or OU_SWEEP_PERSISTENT");
     OU_HIP(hipStreamSynchronize(pl.stream));
-    pl.sweep_mode = mode;
+    ou::set_sweep_mode(pl, mode);
     return int(OU_OK);
   });
 }

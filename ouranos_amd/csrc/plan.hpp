@@ -32,6 +32,15 @@ struct Chunk {
   int cell_offset;  // into the cell list
 };
 
+// The 2D tile sweep (OU_SWEEP_TILES): a tile is a kTileEdge x kTileEdge block
+// of cells of one octant (quadrant) in its canonical frame, canonical cells
+// (kTileEdge ti + a, kTileEdge tj + b), all of one material.  Tiles of tile
+// level ti + tj depend only on tiles of the previous tile level.
+constexpr int kTileEdge = 8;
+struct TileItem {
+  int octant, ti, tj, material;
+};
+
 // Device-side views handed to the kernels (plain pointers, by value).
 struct SweepView {
   int n[3];
@@ -127,8 +136,9 @@ struct Plan {
   }
   long n_chunks = 0;
 
-  // How a sweep is launched: OU_SWEEP_LEVELS (one launch per level) or
-  // OU_SWEEP_PERSISTENT (one persistent launch).
+  // How a sweep is launched: OU_SWEEP_LEVELS (one launch per level),
+  // OU_SWEEP_PERSISTENT (one persistent launch) or OU_SWEEP_TILES (2D: one
+  // launch per tile level).
   int sweep_mode = OU_SWEEP_LEVELS;
   PersistTables deps;                  // upwind chunks of every chunk (host)
   int *d_dep_off = nullptr, *d_dep_list = nullptr;
@@ -137,6 +147,16 @@ struct Plan {
   double wait_limit_s = 10.0;          // a persistent-sweep wait gives up after this long
   double wall_clock_khz = 100000.0;    // rate of the device's wall clock
   int cus = 256;
+
+  // 2D tile sweep (OU_SWEEP_TILES): tiles by tile level, and its own
+  // workgroup shape (waves of 8 directions each, direction groups).
+  bool tiles_ok = false;             // every tile holds one material and a kernel exists
+  std::vector<int> tile_level_off;   // [tile levels + 1] into d_tiles
+  TileItem *d_tiles = nullptr;
+  int tile_waves = 1, tile_groups = 1;
+  unsigned oct_mask = 0;             // octants that hold directions
+  // direction groups of the partial-moment layout the active mode writes
+  int groups_in_use() const { return sweep_mode == OU_SWEEP_TILES ? tile_groups : dir_groups; }
 
   // device buffers
   hipStream_t stream = nullptr;
@@ -177,6 +197,8 @@ struct Plan {
 struct MemberView {
   long n_cells, n_dofs, vec_size;
   int n_mom, L, n_dir, n_parts;  // n_parts = octants x direction groups
+  int groups;                     // direction groups per octant
+  unsigned oct_mask;              // octants that hold directions (others' parts are skipped)
   const int *mat;
   const double *sigma_s;
   const int *mom_l;
@@ -234,6 +256,14 @@ bool sweep_supported(int dim, int p, int mm, int precision, int waves, int max_l
 // octant).  Growing the group count reallocates the partial moments after a
 // device synchronisation.
 void set_sweep_shape(Plan &pl, int waves, int dir_groups);
+// The tile sweep's shape: waves per workgroup (8 directions each) and
+// direction groups per quadrant.
+void set_tile_shape(Plan &pl, int waves, int groups);
+bool tile_sweep_supported(int p, int mm, int precision, int waves, int max_lds);
+// Largest tile-sweep wave count (1, 2 or 4) with a kernel that fits max_lds.
+int tile_waves_max(int p, int mm, int precision, int max_lds);
+// Switches the launch mode (OU_SWEEP_LEVELS, OU_SWEEP_PERSISTENT, OU_SWEEP_TILES).
+void set_sweep_mode(Plan &pl, int mode);
 
 // Launch sequences (stream-ordered, no host synchronisation).
 void launch_build_inverses(Plan &pl);
@@ -248,6 +278,9 @@ void launch_sweep_levels(Plan &pl, const void *S0, hipStream_t s,
                          hipEvent_t before = nullptr, hipEvent_t after = nullptr);
 void launch_epilogue(Plan &pl, const double *d_x, double *d_y, bool apply,
                      hipStream_t s);
+// One launch per tile level (sweep_tile.hip).
+void launch_sweep_tiles(Plan &pl, const SweepView &v, hipStream_t s, hipEvent_t before,
+                        hipEvent_t after);
 
 // Whole operations.
 void apply_device(Plan &pl, const double *d_x, double *d_y, hipStream_t s);

@@ -19,21 +19,11 @@
 
 #include "fe.hpp"
 #include "plan.hpp"
+#include "sweep_common.hpp"
 
 namespace ou {
 
 namespace {
-
-// Read-only data that every lane of a wave indexes identically is loaded
-// through the constant address space, which lets the compiler use scalar
-// loads (one SGPR operand per FMA instead of a vector register).
-template <typename T>
-using cptr = const __attribute__((address_space(4))) T *;
-
-template <typename T>
-__device__ __forceinline__ cptr<T> as_const(const void *p) {
-  return (cptr<T>)(p);
-}
 
 // ------------------------------------------------------------------------
 // Inverse matrices: one 64-lane wave per (material, direction class).  The
@@ -223,25 +213,6 @@ __global__ __launch_bounds__(256) void k_trace_init(T *__restrict__ tr, long per
 // angular flux out) and is built into the stored inverses.  The face mass
 // matrices are unchanged by the reflection (equidistant nodes are symmetric).
 // ------------------------------------------------------------------------
-template <int DIM, int P>
-__device__ __forceinline__ int natural_index(int i, int o) {
-  constexpr int P1 = P + 1;
-  int ix = i % P1, iy = (i / P1) % P1, iz = i / (P1 * P1);
-  if (o & 1) ix = P - ix;
-  if (o & 2) iy = P - iy;
-  if (DIM == 3 && (o & 4)) iz = P - iz;
-  return ix + P1 * (iy + P1 * iz);
-}
-
-template <int DIM, int P, int A, int SIDE>
-__device__ constexpr int face_dof(int fi) {
-  constexpr int P1 = P + 1;
-  const int u = fi % P1, v = fi / P1;
-  if (DIM == 2) return A == 0 ? SIDE + P1 * u : u + P1 * SIDE;
-  return A == 0 ? SIDE + P1 * (u + P1 * v)
-                : (A == 1 ? u + P1 * (SIDE + P1 * v) : u + P1 * (v + P1 * SIDE));
-}
-
 #include "sweep_level.inc"
 
 // Rearranges the canonical row-major inverses of FP32 Q2 hexahedra into the
@@ -277,29 +248,34 @@ __global__ __launch_bounds__(64) void k_ainv_to_mfma64(const double *__restrict_
 
 // y = x - sum_o partial_o (Apply, mode 1), y += sum_o partial_o (sweep, mode
 // 0) or y = 0 + sum_o partial_o (a sweep into a fresh vector, mode 2), summed
-// over the n_parts = octants x direction groups partial moment vectors.
+// over the n_parts = octants x `groups` direction groups partial moment
+// vectors of the octants in oct_mask (an octant without directions has no
+// workgroups, so its slots hold whatever another layout left there).
 // A persistent sweep whose watchdog gave up (error word set) leaves its
 // partial moments incomplete: the output is NaN then, so that no caller
 // mistakes it for a result.
-__device__ __forceinline__ void epilogue_one(long gid, int n_parts, long stride,
+__device__ __forceinline__ void epilogue_one(long gid, int n_parts, int groups,
+                                             unsigned oct_mask, long stride,
                                              const double *__restrict__ partial,
                                              const double *__restrict__ x,
                                              double *__restrict__ y, int apply,
                                              const int *error = nullptr) {
   double s = 0.0;
-  for (int o = 0; o < n_parts; ++o) s += partial[o * stride + gid];
+  for (int k = 0; k < n_parts; ++k)
+    if ((oct_mask >> (k / groups)) & 1u) s += partial[k * stride + gid];
   if (error && *error) s = __builtin_nan("");
   y[gid] = apply == 1 ? x[gid] - s : (apply == 2 ? 0.0 + s : y[gid] + s);
 }
 
-__global__ __launch_bounds__(256) void k_epilogue(long n, int n_parts, long stride,
+__global__ __launch_bounds__(256) void k_epilogue(long n, int n_parts, int groups,
+                                                  unsigned oct_mask, long stride,
                                                   const double *__restrict__ partial,
                                                   const double *__restrict__ x,
                                                   double *__restrict__ y, int apply,
                                                   const int *error) {
   const long gid = long(blockIdx.x) * blockDim.x + threadIdx.x;
   if (gid >= n) return;
-  epilogue_one(gid, n_parts, stride, partial, x, y, apply, error);
+  epilogue_one(gid, n_parts, groups, oct_mask, stride, partial, x, y, apply, error);
 }
 
 // Clears the persistent sweep's completion flags and ticket before its launch
@@ -375,7 +351,8 @@ __global__ __launch_bounds__(256) void k_batch_epilogue(const MemberView *__rest
     double *y = ys[u];
     for (long gid = long(blockIdx.x) * blockDim.x + threadIdx.x; gid < m.vec_size;
          gid += stride)
-      epilogue_one(gid, m.n_parts, m.vec_size, m.partial, x, y, apply, error);
+      epilogue_one(gid, m.n_parts, m.groups, m.oct_mask, m.vec_size, m.partial, x, y, apply,
+                   error);
   }
 }
 
@@ -824,17 +801,22 @@ void launch_sweep_levels(Plan &pl, const void *S0, hipStream_t s, hipEvent_t bef
   const LevelKernels *k = pl.precision == OU_FP32 ? pick<float>(pl.dim, pl.p, pl.mm, pl.waves)
                                                   : pick<double>(pl.dim, pl.p, pl.mm, pl.waves);
   if (!k) fail(OU_ERR_UNSUPPORTED, "no sweep kernel for this element / moment count");
-  if (pl.sweep_mode == OU_SWEEP_PERSISTENT)
+  if (pl.sweep_mode == OU_SWEEP_TILES) {
+    v.dir_groups = pl.tile_groups;
+    launch_sweep_tiles(pl, v, s, before, after);
+  } else if (pl.sweep_mode == OU_SWEEP_PERSISTENT) {
     k->persistent(pl, v, s, before, after);
-  else
+  } else {
     k->single(pl, v, s, before, after);
+  }
 }
 
 void launch_epilogue(Plan &pl, const double *d_x, double *d_y, bool apply, hipStream_t s) {
   const long n = pl.vec_size;
+  const int groups = pl.groups_in_use();
   hipLaunchKernelGGL(k_epilogue, dim3(grid_for(n, 256)), dim3(256), 0, s, n,
-                     pl.n_oct * pl.dir_groups, pl.vec_size, pl.d_partial, d_x, d_y,
-                     int(apply),
+                     pl.n_oct * groups, groups, pl.oct_mask, pl.vec_size, pl.d_partial, d_x,
+                     d_y, int(apply),
                      pl.sweep_mode == OU_SWEEP_PERSISTENT ? pl.d_error : nullptr);
   OU_HIP(hipGetLastError());
 }
