@@ -4,7 +4,10 @@ One operator application of a named workload on device buffers, followed by
 a 1 GiB device copy through the library's plain copy kernel (the reference
 point that scales the FETCH_SIZE / WRITE_SIZE traffic counters).
 
-    python -m ouranos_amd.profile_driver [workload] [fp32|fp64] [levels|persistent]
+    python -m ouranos_amd.profile_driver [workload] [fp32|fp64] [levels|persistent|tiles]
+
+Without a mode the plan's default runs (tiles for 2D Q1-Q2 plans that have
+them, level launches otherwise).
 """
 import ctypes
 import sys
@@ -19,7 +22,8 @@ def main(argv):
     spec = getattr(P, name)()
     op = spec.operator(prec)
     if len(argv) > 3:
-        op.sweep_mode = ou.SWEEP_PERSISTENT if argv[3] == "persistent" else ou.SWEEP_LEVELS
+        op.sweep_mode = {"levels": ou.SWEEP_LEVELS, "persistent": ou.SWEEP_PERSISTENT,
+                         "tiles": ou.SWEEP_TILES}[argv[3]]
     x = P.seeded_vector(spec.size, 1)
     d_x = ou.DeviceBuffer(x.nbytes)
     d_y = ou.DeviceBuffer(x.nbytes)
