@@ -207,6 +207,37 @@ def fp64_leg(spec, d_x, d_y, warmup, steps):
                          "sweep_kernel_ms_per_step": sweep_ms}}
 
 
+def config_leg(name, warmup, steps):
+    """Another configuration's operator application in its plan's default sweep
+    mode (the 2D tile sweep for C1 and C2), FP32 and FP64: ms per application
+    (HIP events) and the sweep kernels' share."""
+    import ouranos_amd as ou
+    from ouranos_amd import problems as P
+    spec, desc = workload(name)
+    log(f"[bench] {name} leg")
+    x = P.seeded_vector(spec.size, 1)
+    d_x = ou.DeviceBuffer(x.nbytes)
+    d_y = ou.DeviceBuffer(x.nbytes)
+    d_x.upload(x)
+    out = {"workload": desc, "n_cells": list(spec.n), "n_dir": spec.quad.n_dir,
+           "updates_per_sweep": spec.updates_per_sweep()}
+    modes = {ou.SWEEP_LEVELS: "levels", ou.SWEEP_PERSISTENT: "persistent", ou.SWEEP_TILES: "tiles"}
+    for prec, pname in ((ou.FP32, "fp32"), (ou.FP64, "fp64")):
+        op = spec.operator(prec)
+        if warmup > 0:
+            op.time_apply(d_x, d_y, warmup, 1)
+        tm = op.time_apply(d_x, d_y, 0, steps)
+        out[pname] = {"mode": modes[op.sweep_mode], "shape": list(op.sweep_shape),
+                      "ms_per_apply": tm["ms_total"] / steps,
+                      "sweep_kernel_ms": tm["ms_sweep_kernel"] / steps,
+                      "launches_per_sweep": tm["n_sweep_launches"],
+                      "updates_per_s": spec.updates_per_sweep() * steps / (tm["ms_total"] / 1000.0)}
+        op.close()
+    d_x.free()
+    d_y.free()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -219,6 +250,8 @@ def main():
     ap.add_argument("--no-solve", action="store_true",
                     help="skip the time-to-tolerance comparison")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 sweep leg")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the C1 / C2 operator timings")
     ap.add_argument("--tol", type=float, default=1e-8)
     a = ap.parse_args()
     if a.gpus != 1:
@@ -284,6 +317,9 @@ def main():
         line["fp64_sweep"] = fp64_leg(spec, d_x, d_y, a.warmup, a.steps)
     d_x.free()
     d_y.free()
+    if not a.no_configs and spec.name == "headline_3d":
+        line["configs"] = {"C1": config_leg("uniform_2d", a.warmup, a.steps),
+                           "C2": config_leg("hetero_2d", a.warmup, a.steps)}
     if not a.no_solve and spec.name == "headline_3d":
         try:
             line["time_to_tol"] = time_to_tolerance(spec, op, a.tol)
