@@ -207,6 +207,27 @@ def fp64_leg(spec, d_x, d_y, warmup, steps):
                          "sweep_kernel_ms_per_step": sweep_ms}}
 
 
+def copy_bandwidth(n_bytes=1 << 30, reps=10):
+    """Device-to-device copy rate of the library's plain copy kernel
+    (ou_memcpy_dtod: n_bytes read + n_bytes written per copy), GB/s: the
+    streaming rate HBM actually delivers on this GPU, beside its 8 TB/s peak."""
+    import ctypes
+    import ouranos_amd as ou
+    lib = ou.lib()
+    lib.ou_memcpy_dtod.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
+    a, b = ou.DeviceBuffer(n_bytes), ou.DeviceBuffer(n_bytes)
+    lib.ou_memcpy_dtod(b.ptr, a.ptr, n_bytes)
+    lib.ou_device_synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        lib.ou_memcpy_dtod(b.ptr, a.ptr, n_bytes)
+    lib.ou_device_synchronize()
+    dt = time.perf_counter() - t0
+    a.free()
+    b.free()
+    return 2.0 * n_bytes * reps / dt / 1e9
+
+
 def config_leg(name, warmup, steps):
     """Another configuration's operator application in its plan's default sweep
     mode (the 2D tile sweep for C1 and C2), FP32 and FP64: ms per application
@@ -313,6 +334,10 @@ def main():
         "event_ms_per_step": tm["ms_total"] / a.steps,
         "plan_seconds": t_plan,
     }
+    copy = copy_bandwidth()
+    line["copy_GBs"] = {"value": copy, "frac_of_copy": achieved / copy,
+                        "what": "ou_memcpy_dtod 1 GiB read + 1 GiB written, 10 copies; the "
+                                "sweep's algorithmic GB/s as a share of this rate"}
     if not a.no_fp64 and prec == ou.FP32:
         line["fp64_sweep"] = fp64_leg(spec, d_x, d_y, a.warmup, a.steps)
     d_x.free()

@@ -100,6 +100,11 @@ def main(argv=None):
     batch.set_streams(2)
     log(f"[batch] graph replay, one launch per level (no octant-half side stream): "
         f"{one_stream_ms:.2f} ms")
+    # every member's levels in one persistent launch inside the graph
+    batch.set_sweep_mode(ou.SWEEP_PERSISTENT)
+    persistent_ms = batch.time_apply(xs, ys, a.warmup, a.steps)
+    batch.set_sweep_mode(ou.SWEEP_LEVELS)
+    log(f"[batch] graph replay, one persistent sweep launch: {persistent_ms:.2f} ms")
 
     # bit-for-bit spot check of two members against their own Apply
     batch.apply_device(xs, ys)
@@ -153,6 +158,7 @@ def main(argv=None):
         "precision": "fp32 sweep, fp64 moments",
         "shape": {"waves": info["waves"], "dir_groups": info["dir_groups"]},
         "ms_graph_one_stream": one_stream_ms,
+        "ms_graph_persistent": persistent_ms,
         "levels_per_batch": info["n_levels"], "chunks_per_batch": info["n_chunks"],
         "ms_graph": graph_ms, "ms_eager": eager_ms, "ms_sequential": seq_ms,
         "updates_per_s": updates / (graph_ms * 1e-3),

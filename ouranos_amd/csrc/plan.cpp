@@ -467,60 +467,66 @@ Plan *create_plan(const ou_problem_desc *desc) {
       for (int o = 0; o < p.n_oct; ++o)
         if (p.oct_off[o + 1] > p.oct_off[o]) p.oct_mask |= 1u << o;
 
-      // ---- 2D tile schedule (OU_SWEEP_TILES) --------------------------------
-      // Tiles of kTileEdge x kTileEdge canonical cells per quadrant, listed by
-      // tile level ti + tj.  Each tile's cells must share one material (its
-      // inverse sits in registers for the whole tile); otherwise, or without
-      // a tile kernel for this element, the plan has no tile mode.
-      const int twm = p.dim == 2 ? tile_waves_max(p.p, p.mm, p.precision, p.max_lds) : 0;
+      // ---- tile schedule (OU_SWEEP_TILES) -------------------------------------
+      // Tiles of 8 x 8 (2D) or 4 x 4 x 4 (3D) canonical cells per octant, listed
+      // by tile level.  Each tile's cells must share one material (its
+      // inverses sit in registers for the whole tile); otherwise, or without a
+      // tile kernel for this element, the plan has no tile mode.
+      const int twm = tile_waves_max(p.dim, p.p, p.mm, p.precision, p.max_lds);
       if (twm > 0) {
-        const int ntx = (nx + kTileEdge - 1) / kTileEdge, nty = (ny + kTileEdge - 1) / kTileEdge;
-        std::vector<std::vector<TileItem>> by_level(size_t(ntx + nty - 1));
+        const int E = p.dim == 2 ? kTileEdge2 : kTileEdge3;
+        const int ntx = (nx + E - 1) / E, nty = (ny + E - 1) / E;
+        const int ntz = p.dim == 3 ? (nz + E - 1) / E : 1;
+        std::vector<std::vector<TileItem>> by_level(size_t(ntx + nty + ntz - 2));
         bool uniform = true;
-        int n_oct_used = 0;
         for (int o = 0; o < p.n_oct && uniform; ++o) {
           if (p.oct_off[o + 1] == p.oct_off[o]) continue;
-          ++n_oct_used;
-          for (int tj = 0; tj < nty && uniform; ++tj)
-            for (int ti = 0; ti < ntx && uniform; ++ti) {
-              int m = -1;
-              for (int b = 0; b < kTileEdge && uniform; ++b)
-                for (int a = 0; a < kTileEdge; ++a) {
-                  const int ii = ti * kTileEdge + a, jj = tj * kTileEdge + b;
-                  if (ii >= nx || jj >= ny) continue;
-                  const int i = (o & 1) ? nx - 1 - ii : ii;
-                  const int j = (o & 2) ? ny - 1 - jj : jj;
-                  const int mc = p.mat[size_t(i) + size_t(nx) * j];
-                  if (m < 0) m = mc;
-                  if (mc != m) {
-                    uniform = false;
-                    break;
-                  }
-                }
-              by_level[size_t(ti + tj)].push_back(TileItem{o, ti, tj, m});
-            }
+          for (int tk = 0; tk < ntz && uniform; ++tk)
+            for (int tj = 0; tj < nty && uniform; ++tj)
+              for (int ti = 0; ti < ntx && uniform; ++ti) {
+                int m = -1;
+                for (int c = 0; c < (p.dim == 3 ? E : 1) && uniform; ++c)
+                  for (int b = 0; b < E && uniform; ++b)
+                    for (int a = 0; a < E; ++a) {
+                      const int ii = ti * E + a, jj = tj * E + b, kk = tk * E + c;
+                      if (ii >= nx || jj >= ny || kk >= nz) continue;
+                      const int i = (o & 1) ? nx - 1 - ii : ii;
+                      const int j = (o & 2) ? ny - 1 - jj : jj;
+                      const int k = (p.dim == 3 && (o & 4)) ? nz - 1 - kk : kk;
+                      const int mc = p.mat[size_t(i) + size_t(nx) * (size_t(j) + size_t(ny) * k)];
+                      if (m < 0) m = mc;
+                      if (mc != m) {
+                        uniform = false;
+                        break;
+                      }
+                    }
+                by_level[size_t(ti + tj + tk)].push_back(TileItem{o, ti, tj, tk, m});
+              }
         }
-        if (uniform && n_oct_used > 0) {
+        if (uniform && p.oct_mask != 0) {
           std::vector<TileItem> tiles;
           p.tile_level_off.assign(1, 0);
+          size_t widest = 1;
           for (const auto &lv : by_level) {
             tiles.insert(tiles.end(), lv.begin(), lv.end());
             p.tile_level_off.push_back(int(tiles.size()));
+            widest = std::max(widest, lv.size());
           }
           p.d_tiles = static_cast<TileItem *>(dev_alloc(tiles.size() * sizeof(TileItem)));
           upload(p.d_tiles, tiles);
           p.tiles_ok = true;
           // About one workgroup per CU at the widest tile level, each with at
           // most kMaxSlotsPerGroup directions, preferably in one round of
-          // 8 directions per wave.
-          const int mid = std::min(ntx, nty) * n_oct_used;
+          // dpw directions per wave.
+          const int dpw = p.dim == 2 ? 8 : 4;
+          const int mid = int(widest);
           int g = std::min(kMaxDirGroups, std::max(1, (p.cus + mid - 1) / mid));
-          g = std::min(g, (max_dirs + 7) / 8);
-          g = std::max(g, (max_dirs + 8 * twm - 1) / (8 * twm));
+          g = std::min(g, (max_dirs + dpw - 1) / dpw);
+          g = std::max(g, (max_dirs + dpw * twm - 1) / (dpw * twm));
           g = std::max(p.min_dir_groups, std::min(g, kMaxDirGroups));
           const int glen = (max_dirs + g - 1) / g;
           p.tile_groups = g;
-          const int need = (glen + 7) / 8;  // waves for one round
+          const int need = (glen + dpw - 1) / dpw;  // waves for one round
           p.tile_waves = std::min(twm, need <= 1 ? 1 : need <= 2 ? 2 : 4);
           p.partial_groups = std::max(p.partial_groups, p.tile_groups);
           p.sweep_mode = OU_SWEEP_TILES;
@@ -688,11 +694,12 @@ static void reserve_partial_groups(Plan &pl, int groups) {
 
 void set_tile_shape(Plan &pl, int waves, int groups) {
   if (!pl.tiles_ok)
-    fail(OU_ERR_UNSUPPORTED, "this plan has no tile sweep (2D, Q1-Q2, one material per 8x8 tile)");
+    fail(OU_ERR_UNSUPPORTED, "this plan has no tile sweep (2D Q1-Q2 with one material per 8x8 "
+                             "tile, or 3D Q1 with one material per 4x4x4 tile)");
   if (groups < std::max(1, pl.min_dir_groups) || groups > kMaxDirGroups)
     fail(OU_ERR_ARG, "direction groups per quadrant must be in [" +
                          std::to_string(std::max(1, pl.min_dir_groups)) + ", 8]");
-  if (!tile_sweep_supported(pl.p, pl.mm, pl.precision, waves, pl.max_lds))
+  if (!tile_sweep_supported(pl.dim, pl.p, pl.mm, pl.precision, waves, pl.max_lds))
     fail(OU_ERR_UNSUPPORTED, "no tile sweep kernel with this many waves for this element");
   reserve_partial_groups(pl, groups);
   pl.tile_waves = waves;
@@ -703,7 +710,8 @@ void set_sweep_mode(Plan &pl, int mode) {
   if (mode != OU_SWEEP_LEVELS && mode != OU_SWEEP_PERSISTENT && mode != OU_SWEEP_TILES)
     fail(OU_ERR_ARG, "sweep mode must be OU_SWEEP_LEVELS, OU_SWEEP_PERSISTENT or OU_SWEEP_TILES");
   if (mode == OU_SWEEP_TILES && !pl.tiles_ok)
-    fail(OU_ERR_UNSUPPORTED, "this plan has no tile sweep (2D, Q1-Q2, one material per 8x8 tile)");
+    fail(OU_ERR_UNSUPPORTED, "this plan has no tile sweep (2D Q1-Q2 with one material per 8x8 "
+                             "tile, or 3D Q1 with one material per 4x4x4 tile)");
   pl.sweep_mode = mode;
 }
 

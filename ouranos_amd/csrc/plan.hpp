@@ -32,13 +32,13 @@ struct Chunk {
   int cell_offset;  // into the cell list
 };
 
-// The 2D tile sweep (OU_SWEEP_TILES): a tile is a kTileEdge x kTileEdge block
-// of cells of one octant (quadrant) in its canonical frame, canonical cells
-// (kTileEdge ti + a, kTileEdge tj + b), all of one material.  Tiles of tile
-// level ti + tj depend only on tiles of the previous tile level.
-constexpr int kTileEdge = 8;
+// The tile sweep (OU_SWEEP_TILES): a tile is a block of cells of one octant in
+// its canonical frame, 8 x 8 in 2D, 4 x 4 x 4 in 3D (edge E: canonical cells
+// (E ti + a, E tj + b[, E tk + c])), all of one material.  Tiles of tile level
+// ti + tj (+ tk) depend only on tiles of the previous tile level.
+constexpr int kTileEdge2 = 8, kTileEdge3 = 4;
 struct TileItem {
-  int octant, ti, tj, material;
+  int octant, ti, tj, tk, material;
 };
 
 // Device-side views handed to the kernels (plain pointers, by value).
@@ -148,8 +148,8 @@ struct Plan {
   double wall_clock_khz = 100000.0;    // rate of the device's wall clock
   int cus = 256;
 
-  // 2D tile sweep (OU_SWEEP_TILES): tiles by tile level, and its own
-  // workgroup shape (waves of 8 directions each, direction groups).
+  // Tile sweep (OU_SWEEP_TILES): tiles by tile level, and its own workgroup
+  // shape (waves of 8 directions each in 2D, 4 in 3D; direction groups).
   bool tiles_ok = false;             // every tile holds one material and a kernel exists
   std::vector<int> tile_level_off;   // [tile levels + 1] into d_tiles
   TileItem *d_tiles = nullptr;
@@ -259,9 +259,9 @@ void set_sweep_shape(Plan &pl, int waves, int dir_groups);
 // The tile sweep's shape: waves per workgroup (8 directions each) and
 // direction groups per quadrant.
 void set_tile_shape(Plan &pl, int waves, int groups);
-bool tile_sweep_supported(int p, int mm, int precision, int waves, int max_lds);
+bool tile_sweep_supported(int dim, int p, int mm, int precision, int waves, int max_lds);
 // Largest tile-sweep wave count (1, 2 or 4) with a kernel that fits max_lds.
-int tile_waves_max(int p, int mm, int precision, int max_lds);
+int tile_waves_max(int dim, int p, int mm, int precision, int max_lds);
 // Switches the launch mode (OU_SWEEP_LEVELS, OU_SWEEP_PERSISTENT, OU_SWEEP_TILES).
 void set_sweep_mode(Plan &pl, int mode);
 

@@ -29,68 +29,102 @@ namespace ou {
 
 namespace {
 
-constexpr int kTE = kTileEdge;
-constexpr int kDirsPerWave = 64 / kTE;
+// Tile geometry per dimension: edge, lanes (positions) per direction, directions
+// per wave, cell levels inside a tile.
+template <int DIM>
+struct TileGeom {
+  static constexpr int TE = DIM == 2 ? kTileEdge2 : kTileEdge3;
+  static constexpr int PPD = DIM == 2 ? TE : TE * TE;
+  static constexpr int DPW = 64 / PPD;
+  static constexpr int NL = DIM * (TE - 1) + 1;
+};
 
-template <typename T, int P, int MM, int W>
+template <typename T, int DIM, int P, int MM, int W>
 struct TileCfg {
-  static constexpr int P1 = P + 1, N = P1 * P1, F = P1;
+  using G = TileGeom<DIM>;
+  static constexpr int P1 = P + 1, N = DIM == 2 ? P1 * P1 : P1 * P1 * P1;
+  static constexpr int F = DIM == 2 ? P1 : P1 * P1;
   // LDS stride of one cell, padded to 16 bytes for 128-bit LDS accesses
   static constexpr int NP = sizeof(T) == 4 ? (N + 3) & ~3 : (N + 1) & ~1;
   static constexpr int NC = (N + W - 1) / W;  // (cell, dof) columns per lane in D2M
-  static constexpr size_t psi_bytes = size_t(W) * kDirsPerWave * 64 * NP * sizeof(T);
+  static constexpr size_t psi_bytes = size_t(W) * G::DPW * 64 * NP * sizeof(T);
   static constexpr size_t src_bytes = size_t(MM) * 64 * NP * sizeof(T);
   static constexpr size_t d2m_bytes = size_t(kMaxSlotsPerGroup) * MM * sizeof(double);
   static constexpr size_t lds_bytes = psi_bytes + src_bytes + d2m_bytes;
 };
 
-// x of lane l - 1 within each row of 16 lanes (DPP row_shr:1); a lane at the
-// start of a row keeps its own value.
-__device__ __forceinline__ float lane_shr1(float x) {
-  const int b = __float_as_int(x);
-  return __int_as_float(__builtin_amdgcn_update_dpp(b, b, 0x111, 0xF, 0xF, false));
+// x of lane l - S within each row of 16 lanes (DPP row_shr:S); a lane whose
+// source would fall before its row's start keeps its own value.
+template <int S>
+__device__ __forceinline__ int lane_shr_bits(int b) {
+  return __builtin_amdgcn_update_dpp(b, b, 0x110 + S, 0xF, 0xF, false);
 }
 
-__device__ __forceinline__ double lane_shr1(double x) {
+template <int S>
+__device__ __forceinline__ float lane_shr(float x) {
+  return __int_as_float(lane_shr_bits<S>(__float_as_int(x)));
+}
+
+template <int S>
+__device__ __forceinline__ double lane_shr(double x) {
   const unsigned long long b = static_cast<unsigned long long>(__double_as_longlong(x));
-  const int lo = int(unsigned(b)), hi = int(unsigned(b >> 32));
-  const unsigned lo2 = unsigned(__builtin_amdgcn_update_dpp(lo, lo, 0x111, 0xF, 0xF, false));
-  const unsigned hi2 = unsigned(__builtin_amdgcn_update_dpp(hi, hi, 0x111, 0xF, 0xF, false));
-  return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi2) << 32) | lo2));
+  const unsigned lo = unsigned(lane_shr_bits<S>(int(unsigned(b))));
+  const unsigned hi = unsigned(lane_shr_bits<S>(int(unsigned(b >> 32))));
+  return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
 }
 
-template <typename T, int P, int MM, int W>
-__global__ __launch_bounds__(64 * W) void k_sweep_tile(SweepView v,
-                                                       const TileItem *__restrict__ tiles,
-                                                       int tile_begin) {
-  using C = TileCfg<T, P, MM, W>;
+// out = cand[k] for the lane's k (0 <= k < K), zeros otherwise: a select
+// chain, with no register array indexed by a lane-dependent value.
+template <typename T, int K, int F>
+__device__ __forceinline__ void select_of(const T (&cand)[K][F], int k, T *out) {
+#pragma unroll
+  for (int f = 0; f < F; ++f) out[f] = T(0);
+#pragma unroll
+  for (int j = 0; j < K; ++j)
+#pragma unroll
+    for (int f = 0; f < F; ++f) out[f] = k == j ? cand[j][f] : out[f];
+}
+
+// One tile of one octant for direction group `group` (see the file comment).
+// 2D: lane = 8 x direction + a, cell (a, t - a) at tile level t; x-inflow
+// from lane - 1, y-inflow the lane's own previous cell.  3D: lane = 16 x
+// direction + 4 b + a, cell (a, b, t - a - b); x-inflow from lane - 1,
+// y-inflow from lane - 4, z-inflow the lane's own previous cell.
+template <typename T, int DIM, int P, int MM, int W>
+__device__ __forceinline__ void sweep_tile_item(const SweepView &v, const TileItem tile,
+                                                int group) {
+  using C = TileCfg<T, DIM, P, MM, W>;
+  using G = TileGeom<DIM>;
   constexpr int N = C::N, F = C::F, NP = C::NP, NC = C::NC;
+  constexpr int TE = G::TE, PPD = G::PPD, DPW = G::DPW, NL = G::NL;
   extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
-  T *s_psi = reinterpret_cast<T *>(lds_raw);                      // [W*8 slots][64 cells][NP]
+  T *s_psi = reinterpret_cast<T *>(lds_raw);                      // [W*DPW slots][64 cells][NP]
   T *s_src = reinterpret_cast<T *>(lds_raw + C::psi_bytes);       // [MM][64 cells][NP]
   double *t_d2m = reinterpret_cast<double *>(lds_raw + C::psi_bytes + C::src_bytes);  // [slot][MM]
 
-  const TileItem tile = tiles[tile_begin + blockIdx.x];
-  const int group = blockIdx.y;
   const int tid = threadIdx.x, lane = tid & 63;
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int o = tile.octant;
-  const int nx = v.n[0], ny = v.n[1];
+  const int nx = v.n[0], ny = v.n[1], nz = v.n[2];
   const int n_groups = v.dir_groups, dall = v.oct_off[o + 1] - v.oct_off[o];
   const int glen = (dall + n_groups - 1) / n_groups;
   const int dbeg = v.oct_off[o] + group * glen;
   const int ndir_o = max(0, min(glen, dall - group * glen));
-  const int ii0 = tile.ti * kTE, jj0 = tile.tj * kTE;
+  const int ii0 = tile.ti * TE, jj0 = tile.tj * TE, kk0 = tile.tk * TE;
   const long n_dofs = v.n_dofs;
   const int n_mom = v.n_mom;
 
-  // Tile cell cl = kTE b + a: canonical (ii0 + a, jj0 + b), -1 outside the mesh.
+  // Tile cell cl = pos + PPD e (pos = a + TE b in 3D): canonical
+  // (ii0 + a, jj0 + b[, kk0 + c]), -1 outside the mesh.
   auto cell_of = [&](int cl) {
-    const int ii = ii0 + cl % kTE, jj = jj0 + cl / kTE;
-    if (ii >= nx || jj >= ny) return -1;
+    const int ii = ii0 + cl % TE;
+    const int jj = jj0 + (cl / TE) % TE;
+    const int kk = DIM == 3 ? kk0 + cl / (TE * TE) : 0;
+    if (ii >= nx || jj >= ny || kk >= nz) return -1;
     const int i = (o & 1) ? nx - 1 - ii : ii;
     const int j = (o & 2) ? ny - 1 - jj : jj;
-    return i + nx * j;
+    const int k = (DIM == 3 && (o & 4)) ? nz - 1 - kk : kk;
+    return i + nx * (j + ny * k);
   };
 
   // ---- stage the group's D2M columns and the tile's source ----------------
@@ -131,7 +165,7 @@ __global__ __launch_bounds__(64 * W) void k_sweep_tile(SweepView v,
 #pragma unroll
           for (int j = 0; j < Q; ++j)
             if (q * Q + j < N)
-              s_src[m * 64 * NP + cl * NP + natural_index<2, P>(q * Q + j, o)] = val[k][j];
+              s_src[m * 64 * NP + cl * NP + natural_index<DIM, P>(q * Q + j, o)] = val[k][j];
         }
       }
     }
@@ -139,11 +173,11 @@ __global__ __launch_bounds__(64 * W) void k_sweep_tile(SweepView v,
   lds_barrier();
 
   const T *m1 = m1_of<T>(v);
-  const int a = lane % kTE, dsub = lane / kTE;
-  const bool col_in = ii0 + a < nx;
-  T *const tr0 = static_cast<T *>(v.trace[0]);
-  T *const tr1 = static_cast<T *>(v.trace[1]);
-  const long ps0 = v.plane_size[0], ps1 = v.plane_size[1];
+  const int pos = lane % PPD, dsub = lane / PPD;
+  const int a = pos % TE, b = DIM == 3 ? pos / TE : 0;  // 2D: b is the level's row
+  const int ii = ii0 + a, jj3 = jj0 + b;
+  const bool col_in = ii < nx && (DIM == 2 || jj3 < ny);
+  const long ps0 = v.plane_size[0], ps1 = v.plane_size[1], ps2 = v.plane_size[2];
 
   double acc[MM][NC];
 #pragma unroll
@@ -151,8 +185,8 @@ __global__ __launch_bounds__(64 * W) void k_sweep_tile(SweepView v,
 #pragma unroll
     for (int t = 0; t < NC; ++t) acc[m][t] = 0.0;
 
-  for (int r0 = 0; r0 < ndir_o; r0 += W * kDirsPerWave) {
-    const int sl = r0 + w * kDirsPerWave + dsub;  // slot within the group
+  for (int r0 = 0; r0 < ndir_o; r0 += W * DPW) {
+    const int sl = r0 + w * DPW + dsub;  // slot within the group
     const bool has_dir = sl < ndir_o;
     const int gsl = dbeg + (has_dir ? sl : 0);
     const int d = v.oct_dirs[gsl];
@@ -161,52 +195,105 @@ __global__ __launch_bounds__(64 * W) void k_sweep_tile(SweepView v,
 #pragma unroll
     for (int m = 0; m < MM; ++m)
       m2d[m] = has_dir ? static_cast<const T *>(v.slot_m2d)[long(gsl) * MM + m] : T(0);
-    const T c0 = static_cast<const T *>(v.slot_c)[long(gsl) * 4 + 0];
-    const T c1 = static_cast<const T *>(v.slot_c)[long(gsl) * 4 + 1];
+    T cf[DIM];
+#pragma unroll
+    for (int x = 0; x < DIM; ++x) cf[x] = static_cast<const T *>(v.slot_c)[long(gsl) * 4 + x];
     // the (tile material, class) inverse, canonical frame, in registers
     const T *Ai = static_cast<const T *>(v.ainv) + (long(tile.material) * v.n_cls + cls) * (N * N);
     T A[N * N];
 #pragma unroll
     for (int k = 0; k < N * N; ++k) A[k] = Ai[k];
-    // Trace slabs of direction d: x slot = canonical row jj, y slot = column ii.
-    T *const dx = tr0 + long(d) * F * ps0;
-    T *const dy = tr1 + long(d) * F * ps1;
-    // upwind edge of the tile: rows jj0 + b (lanes a = 0), column ii0 + a
-    T xin[kTE][F], yin[F];
+    // trace slabs of direction d
+    T *const dx = static_cast<T *>(v.trace[0]) + long(d) * F * ps0;
+    T *const dy = static_cast<T *>(v.trace[1]) + long(d) * F * ps1;
+    T *const dz = DIM == 3 ? static_cast<T *>(v.trace[2]) + long(d) * F * ps2 : nullptr;
+
+    // Upwind edges of the tile.  2D: rows jj0 + b (lanes a = 0, row = level),
+    // column ii0 + a.  3D: x-face cells (0, b, c) and y-face cells (a, 0, c)
+    // for c = 0..3 (the level picks c), z-face cell (a, b, 0).
+    T xin[TE][F], yin[DIM == 2 ? 1 : TE][F], zin[F];
+    if constexpr (DIM == 2) {
 #pragma unroll
-    for (int b = 0; b < kTE; ++b)
+      for (int r = 0; r < TE; ++r)
+#pragma unroll
+        for (int f = 0; f < F; ++f)
+          xin[r][f] = (a == 0 && jj0 + r < ny) ? dx[long(jj0 + r) * F + f] : T(0);
+#pragma unroll
+      for (int f = 0; f < F; ++f) yin[0][f] = col_in ? dy[long(ii) * F + f] : T(0);
+    } else {
+#pragma unroll
+      for (int c = 0; c < TE; ++c) {
+        const int kk = kk0 + c;
+#pragma unroll
+        for (int f = 0; f < F; ++f) {
+          xin[c][f] = (a == 0 && jj3 < ny && kk < nz) ? dx[(long(jj3) + long(ny) * kk) * F + f] : T(0);
+          yin[c][f] = (b == 0 && ii < nx && kk < nz) ? dy[(long(ii) + long(nx) * kk) * F + f] : T(0);
+        }
+      }
 #pragma unroll
       for (int f = 0; f < F; ++f)
-        xin[b][f] = (a == 0 && jj0 + b < ny) ? dx[long(jj0 + b) * F + f] : T(0);
-#pragma unroll
-    for (int f = 0; f < F; ++f) yin[f] = col_in ? dy[long(ii0 + a) * F + f] : T(0);
+        zin[f] = col_in ? dz[(long(ii + jj3) * ny + jj3) * F + f] : T(0);
+    }
 
-    T tx[F], ty[F];
+    // M2D source term of the lane's cell at tile level t (clamped to the
+    // tile for lanes outside it at that level)
+    auto source_term = [&](int t, T *out) {
+      const int e = t - (DIM == 2 ? a : a + b);
+      const int cl = pos + PPD * min(max(e, 0), TE - 1);
 #pragma unroll
-    for (int f = 0; f < F; ++f) tx[f] = ty[f] = T(0);
-#pragma unroll
-    for (int t = 0; t < 2 * kTE - 1; ++t) {
-      const int b = t - a;
-      const int bc = min(max(b, 0), kTE - 1);
-      const int cl = bc * kTE + a;
-      const bool active = has_dir && b >= 0 && b < kTE && col_in && jj0 + b < ny;
-      T xi[F], yi[F];
-#pragma unroll
-      for (int f = 0; f < F; ++f) {
-        xi[f] = a == 0 ? (t < kTE ? xin[t < kTE ? t : 0][f] : T(0)) : tx[f];
-        yi[f] = b == 0 ? yin[f] : ty[f];
-      }
-      T bv[N];
-#pragma unroll
-      for (int i = 0; i < N; ++i) bv[i] = T(0);
+      for (int i = 0; i < N; ++i) out[i] = T(0);
 #pragma unroll
       for (int m = 0; m < MM; ++m) {
         const T *src = s_src + m * 64 * NP + cl * NP;
 #pragma unroll
-        for (int i = 0; i < N; ++i) bv[i] += m2d[m] * src[i];
+        for (int i = 0; i < N; ++i) out[i] += m2d[m] * src[i];
       }
-      add_upwind<T, 2, P, 0>(m1, c0, xi, bv);
-      add_upwind<T, 2, P, 1>(m1, c1, yi, bv);
+    };
+    // FP32 with up to 10 moments forms it a level ahead (the registers allow)
+    constexpr bool ahead = sizeof(T) == 4 && MM <= 10;
+    T tx[F], ty[F], tz[F], bnext[N];
+#pragma unroll
+    for (int f = 0; f < F; ++f) tx[f] = ty[f] = tz[f] = T(0);
+    if constexpr (ahead) source_term(0, bnext);
+#pragma unroll
+    for (int t = 0; t < NL; ++t) {
+      const int e = t - (DIM == 2 ? a : a + b);  // the last coordinate of the lane's cell
+      const int ec = min(max(e, 0), TE - 1);
+      const int cl = pos + PPD * ec;
+      T xi[F], yi[F], zi[F];
+      bool active;
+      if constexpr (DIM == 2) {
+        active = has_dir && e >= 0 && e < TE && col_in && jj0 + e < ny;
+#pragma unroll
+        for (int f = 0; f < F; ++f) {
+          xi[f] = a == 0 ? (t < TE ? xin[t < TE ? t : 0][f] : T(0)) : tx[f];
+          yi[f] = e == 0 ? yin[0][f] : ty[f];
+        }
+      } else {
+        active = has_dir && e >= 0 && e < TE && col_in && kk0 + e < nz;
+        T xs[F], ys[F];
+        select_of<T, TE, F>(xin, e, xs);
+        select_of<T, TE, F>(yin, e, ys);
+#pragma unroll
+        for (int f = 0; f < F; ++f) {
+          xi[f] = a == 0 ? xs[f] : tx[f];
+          yi[f] = b == 0 ? ys[f] : ty[f];
+          zi[f] = e == 0 ? zin[f] : tz[f];
+        }
+      }
+      // the source term depends on no earlier cell: formed one level ahead it
+      // stays off the chain of dependent cell solves
+      T bv[N];
+      if constexpr (ahead) {
+#pragma unroll
+        for (int i = 0; i < N; ++i) bv[i] = bnext[i];
+        if (t + 1 < NL) source_term(t + 1, bnext);
+      } else {
+        source_term(t, bv);
+      }
+      add_upwind<T, DIM, P, 0>(m1, cf[0], xi, bv);
+      add_upwind<T, DIM, P, 1>(m1, cf[1], yi, bv);
+      if constexpr (DIM == 3) add_upwind<T, DIM, P, 2>(m1, cf[2], zi, bv);
       T psi[N];
 #pragma unroll
       for (int r = 0; r < N; ++r) {
@@ -216,18 +303,34 @@ __global__ __launch_bounds__(64 * W) void k_sweep_tile(SweepView v,
         psi[r] = s;
       }
       if (active) {
-        T *dst = s_psi + ((w * kDirsPerWave + dsub) * 64 + cl) * NP;
+        T *dst = s_psi + ((w * DPW + dsub) * 64 + cl) * NP;
 #pragma unroll
         for (int i = 0; i < N; ++i) dst[i] = psi[i];
-        // downwind edges of the tile (or of the mesh) go to the trace slabs
-        if (a == kTE - 1 || ii0 + a == nx - 1) store_trace<T, 2, P, 0>(dx, unsigned(jj0 + b), psi);
-        if (b == kTE - 1 || jj0 + b == ny - 1) store_trace<T, 2, P, 1>(dy, unsigned(ii0 + a), psi);
+        // downwind faces of the tile (or of the mesh) go to the trace slabs
+        if constexpr (DIM == 2) {
+          if (a == TE - 1 || ii == nx - 1) store_trace<T, 2, P, 0>(dx, unsigned(jj0 + e), psi);
+          if (e == TE - 1 || jj0 + e == ny - 1) store_trace<T, 2, P, 1>(dy, unsigned(ii), psi);
+        } else {
+          const int kk = kk0 + e;
+          if (a == TE - 1 || ii == nx - 1)
+            store_trace<T, 3, P, 0>(dx, unsigned(jj3 + ny * kk), psi);
+          if (b == TE - 1 || jj3 == ny - 1)
+            store_trace<T, 3, P, 1>(dy, unsigned(ii + nx * kk), psi);
+          if (e == TE - 1 || kk == nz - 1)
+            store_trace<T, 3, P, 2>(dz, unsigned((ii + jj3) * ny + jj3), psi);
+        }
       }
-      // next level: x-outflow to the lane of column a + 1, y-outflow kept
+      // next level: x-outflow to the lane of a + 1, 3D y-outflow to the lane
+      // of b + 1, the last axis's outflow kept
 #pragma unroll
       for (int f = 0; f < F; ++f) {
-        tx[f] = lane_shr1(psi[face_dof<2, P, 0, P>(f)]);
-        ty[f] = psi[face_dof<2, P, 1, P>(f)];
+        tx[f] = lane_shr<1>(psi[face_dof<DIM, P, 0, P>(f)]);
+        if constexpr (DIM == 2) {
+          ty[f] = psi[face_dof<DIM, P, 1, P>(f)];
+        } else {
+          ty[f] = lane_shr<TE>(psi[face_dof<DIM, P, 1, P>(f)]);
+          tz[f] = psi[face_dof<DIM, P, 2, P>(f)];
+        }
       }
     }
     lds_barrier();
@@ -240,7 +343,7 @@ __global__ __launch_bounds__(64 * W) void k_sweep_tile(SweepView v,
       const int cl = col / N;
       col_off[t] = col < 64 * N ? cl * NP + (col - cl * N) : 0;
     }
-    const int n_round = min(W * kDirsPerWave, ndir_o - r0);
+    const int n_round = min(W * DPW, ndir_o - r0);
 #pragma unroll 1
     for (int s = 0; s < n_round; ++s) {
       double dm[MM];
@@ -265,7 +368,7 @@ __global__ __launch_bounds__(64 * W) void k_sweep_tile(SweepView v,
       const int c = cell_of(cl);
       if (c >= 0) {
         double *out = v.partial + (long(o) * n_groups + group) * n_mom * n_dofs + long(c) * N +
-                      natural_index<2, P>(ic, o);
+                      natural_index<DIM, P>(ic, o);
 #pragma unroll
         for (int m = 0; m < MM; ++m)
           if (m < n_mom) out[long(m) * n_dofs] = acc[m][t];
@@ -274,12 +377,19 @@ __global__ __launch_bounds__(64 * W) void k_sweep_tile(SweepView v,
   }
 }
 
+template <typename T, int DIM, int P, int MM, int W>
+__global__ __launch_bounds__(64 * W) void k_sweep_tile(SweepView v,
+                                                       const TileItem *__restrict__ tiles,
+                                                       int tile_begin) {
+  sweep_tile_item<T, DIM, P, MM, W>(v, tiles[tile_begin + blockIdx.x], blockIdx.y);
+}
+
 using TileLaunch = void (*)(const Plan &, const SweepView &, hipStream_t);
 
-template <typename T, int P, int MM, int W>
+template <typename T, int DIM, int P, int MM, int W>
 void launch_tiles_impl(const Plan &pl, const SweepView &v, hipStream_t s) {
-  using C = TileCfg<T, P, MM, W>;
-  auto kern = k_sweep_tile<T, P, MM, W>;
+  using C = TileCfg<T, DIM, P, MM, W>;
+  auto kern = k_sweep_tile<T, DIM, P, MM, W>;
   static bool attr_set = false;
   if (!attr_set) {
     OU_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
@@ -295,57 +405,67 @@ void launch_tiles_impl(const Plan &pl, const SweepView &v, hipStream_t s) {
   }
 }
 
-template <typename T, int P, int W>
+template <typename T, int DIM, int P, int W>
 TileLaunch pick_tile_mm(int mm) {
   switch (mm) {
-    case 1: return launch_tiles_impl<T, P, 1, W>;
-    case 4: return launch_tiles_impl<T, P, 4, W>;
-    case 10: return launch_tiles_impl<T, P, 10, W>;
-    case 16: return launch_tiles_impl<T, P, 16, W>;
+    case 1: return launch_tiles_impl<T, DIM, P, 1, W>;
+    case 4: return launch_tiles_impl<T, DIM, P, 4, W>;
+    case 10: if constexpr (DIM == 2) return launch_tiles_impl<T, DIM, P, 10, W>; break;
+    case 16: if constexpr (DIM == 2) return launch_tiles_impl<T, DIM, P, 16, W>; break;
+  }
+  return nullptr;
+}
+
+// Instantiated: 2D Q1-Q2 (up to 16 moments) and 3D Q1 (up to 4 moments), at
+// 1, 2 and 4 waves.
+template <typename T>
+TileLaunch pick_tile(int dim, int p, int mm, int waves) {
+  if (dim == 2 && p == 1) {
+    if (waves == 1) return pick_tile_mm<T, 2, 1, 1>(mm);
+    if (waves == 2) return pick_tile_mm<T, 2, 1, 2>(mm);
+    if (waves == 4) return pick_tile_mm<T, 2, 1, 4>(mm);
+  } else if (dim == 2 && p == 2) {
+    if (waves == 1) return pick_tile_mm<T, 2, 2, 1>(mm);
+    if (waves == 2) return pick_tile_mm<T, 2, 2, 2>(mm);
+    if (waves == 4) return pick_tile_mm<T, 2, 2, 4>(mm);
+  } else if (dim == 3 && p == 1) {
+    if (waves == 1) return pick_tile_mm<T, 3, 1, 1>(mm);
+    if (waves == 2) return pick_tile_mm<T, 3, 1, 2>(mm);
+    if (waves == 4) return pick_tile_mm<T, 3, 1, 4>(mm);
   }
   return nullptr;
 }
 
 template <typename T>
-TileLaunch pick_tile(int p, int mm, int waves) {
-  if (p == 1) {
-    if (waves == 1) return pick_tile_mm<T, 1, 1>(mm);
-    if (waves == 2) return pick_tile_mm<T, 1, 2>(mm);
-    if (waves == 4) return pick_tile_mm<T, 1, 4>(mm);
-  } else if (p == 2) {
-    if (waves == 1) return pick_tile_mm<T, 2, 1>(mm);
-    if (waves == 2) return pick_tile_mm<T, 2, 2>(mm);
-    if (waves == 4) return pick_tile_mm<T, 2, 4>(mm);
-  }
-  return nullptr;
-}
-
-template <typename T>
-size_t tile_lds(int p, int mm, int waves) {
-  const int P1 = p + 1, N = P1 * P1;
+size_t tile_lds(int dim, int p, int mm, int waves) {
+  const int P1 = p + 1, N = dim == 2 ? P1 * P1 : P1 * P1 * P1;
   const int NP = sizeof(T) == 4 ? (N + 3) & ~3 : (N + 1) & ~1;
-  return size_t(waves) * kDirsPerWave * 64 * NP * sizeof(T) + size_t(mm) * 64 * NP * sizeof(T) +
+  const int dpw = dim == 2 ? TileGeom<2>::DPW : TileGeom<3>::DPW;
+  return size_t(waves) * dpw * 64 * NP * sizeof(T) + size_t(mm) * 64 * NP * sizeof(T) +
          size_t(kMaxSlotsPerGroup) * mm * sizeof(double);
 }
 
 }  // namespace
 
-bool tile_sweep_supported(int p, int mm, int precision, int waves, int max_lds) {
+bool tile_sweep_supported(int dim, int p, int mm, int precision, int waves, int max_lds) {
   if (precision == OU_FP32)
-    return pick_tile<float>(p, mm, waves) != nullptr && tile_lds<float>(p, mm, waves) <= size_t(max_lds);
-  return pick_tile<double>(p, mm, waves) != nullptr && tile_lds<double>(p, mm, waves) <= size_t(max_lds);
+    return pick_tile<float>(dim, p, mm, waves) != nullptr &&
+           tile_lds<float>(dim, p, mm, waves) <= size_t(max_lds);
+  return pick_tile<double>(dim, p, mm, waves) != nullptr &&
+         tile_lds<double>(dim, p, mm, waves) <= size_t(max_lds);
 }
 
-int tile_waves_max(int p, int mm, int precision, int max_lds) {
+int tile_waves_max(int dim, int p, int mm, int precision, int max_lds) {
   for (int w : {4, 2, 1})
-    if (tile_sweep_supported(p, mm, precision, w, max_lds)) return w;
+    if (tile_sweep_supported(dim, p, mm, precision, w, max_lds)) return w;
   return 0;
 }
 
 void launch_sweep_tiles(Plan &pl, const SweepView &v, hipStream_t s, hipEvent_t before,
                         hipEvent_t after) {
-  const TileLaunch k = pl.precision == OU_FP32 ? pick_tile<float>(pl.p, pl.mm, pl.tile_waves)
-                                               : pick_tile<double>(pl.p, pl.mm, pl.tile_waves);
+  const TileLaunch k = pl.precision == OU_FP32
+                           ? pick_tile<float>(pl.dim, pl.p, pl.mm, pl.tile_waves)
+                           : pick_tile<double>(pl.dim, pl.p, pl.mm, pl.tile_waves);
   if (!k || !pl.tiles_ok) fail(OU_ERR_UNSUPPORTED, "no tile sweep for this plan");
   if (before) OU_HIP(hipEventRecord(before, s));
   k(pl, v, s);

@@ -147,6 +147,11 @@ def test_c3_batched_instances_full_size(oracle, precision):
           f"{info['n_levels']} merged levels", flush=True)
     xs = [P.seeded_vector(s.size, 500 + i) for i, s in enumerate(specs)]
     ys = batch.apply(xs)
+    # The batch runs the members' chunk schedule at its own shape; a member's
+    # own level launches at that shape run the same arithmetic.  (The members'
+    # default tile sweep sums the directions in other groups.)
+    for op in ops:
+        op.sweep_mode = ou.SWEEP_LEVELS
     for i, (op, x, y) in enumerate(zip(ops, xs, ys)):
         assert np.array_equal(y, op.Apply(x)), i
     print("[gpu] C3 batch: every member equals its single-plan Apply bit-for-bit", flush=True)

@@ -1,5 +1,6 @@
-"""The 2D tile sweep (OU_SWEEP_TILES): one launch per tile level of 8x8-cell
-tiles, the traces of a tile's 15 cell levels kept in registers.
+"""The tile sweep (OU_SWEEP_TILES): one launch per tile level of 8x8-cell
+(2D) or 4x4x4-cell (3D Q1) tiles, the traces of a tile's 15 (10) cell levels
+kept in registers.
 
 Bar: the per-cell arithmetic and the D2M accumulation order are the level
 sweep's, so at equal direction groups Apply and the source sweep equal the
@@ -44,6 +45,12 @@ def _cases():
         "hetero_2d_q2_L3": lambda: P.hetero_2d(n=32, block=8, n_polar=4, n_azim=8, L=3),
         "hetero_2d_q2_L1_64dirs": lambda: P.hetero_2d(n=24, block=8, n_polar=16, n_azim=32, L=1),
         "hetero_2d_q1_L2": lambda: P.hetero_2d(n=16, block=8, n_polar=4, n_azim=8, L=2, p=1),
+        # 3D Q1: LS-S8 (10 per octant) on 4^3 material blocks, LS-S12 (21 per
+        # octant) on 8^3 blocks, one material with partial tiles (4 + 4 + 2)
+        "thick_3d_q1_s8": lambda: P.thick_3d(n=8, block=4),
+        "batch_3d_q1_s12": lambda: P.batch_3d(n=12, block=4),
+        "batch_3d_10_one_block": lambda: P.batch_3d(n=10, block=16),
+        "headline_3d_q1_L1": lambda: P.headline_3d(n=8, block=4, n_polar=4, n_azim=8, L=1, p=1),
     }
 
 
@@ -112,6 +119,46 @@ def test_every_tile_shape_equals_levels(precision, pname):
     op.close()
 
 
+@pytest.mark.parametrize("precision,pname", PRECISIONS, ids=["fp64", "fp32"])
+def test_every_3d_tile_shape_equals_levels(precision, pname):
+    """3D Q1 tiles at waves 1, 2, 4 (4 directions each) x groups 1, 2, 5:
+    several rounds where the waves hold fewer than the group's directions."""
+    spec = P.batch_3d(n=8, block=4)  # LS-S12: 21 directions per octant
+    op = spec.operator(precision)
+    assert op.sweep_mode == ou.SWEEP_TILES
+    x = P.seeded_vector(spec.size, 93)
+    lev = {}
+    op.sweep_mode = ou.SWEEP_LEVELS
+    for g in (1, 2, 5):
+        op.sweep_shape = (4, g)
+        lev[g] = op.Apply(x)
+    op.sweep_mode = ou.SWEEP_TILES
+    for w in (1, 2, 4):
+        for g in (1, 2, 5):
+            print(f"[gpu] 3D tile shape ({w},{g}) {pname}", flush=True)
+            op.sweep_shape = (w, g)
+            assert np.array_equal(op.Apply(x), lev[g]), (w, g)
+    op.close()
+
+
+def test_c4_full_size_tiles(oracle):
+    """C4 (32^3 Q1, LS-S8, 4^3 blocks) in the 3D tile sweep: FP64 at 1e-12
+    against the oracle, and the level launches' bits at the same groups."""
+    spec = P.thick_3d()
+    x = P.seeded_vector(spec.size, 41)
+    ref = oracle_problem(oracle, spec).apply(x)
+    for precision in (ou.FP64, ou.FP32):
+        op = spec.operator(precision)
+        assert op.sweep_mode == ou.SWEEP_TILES
+        print(f"[gpu] C4 tiles shape={op.sweep_shape}", flush=True)
+        y = op.Apply(x)
+        tol = FP64_TOL if precision == ou.FP64 else 2e-5
+        assert rel_err(x - y, x - ref) < tol
+        y_lev, _ = _same_groups_levels(op, x, x)
+        assert np.array_equal(y, y_lev)
+        op.close()
+
+
 def test_quadrant_without_directions(oracle):
     """No direction in the (-,-) quadrant: no tiles for it, and its
     partial-moment slots are skipped, whatever a previous layout left there."""
@@ -134,11 +181,12 @@ def test_quadrant_without_directions(oracle):
 
 
 def test_plans_without_tiles_refuse_the_mode():
-    """3D plans, Q3+ elements and tiles that mix materials have no tile sweep:
-    they start in level launches and refuse OU_SWEEP_TILES."""
+    """3D Q2+, 2D Q3+ and tiles that mix materials have no tile sweep: they
+    start in level launches and refuse OU_SWEEP_TILES."""
     for spec in (P.hetero_2d(n=16, block=4, n_polar=4, n_azim=8, L=1),   # 4x4 material blocks
                  P.hetero_2d(n=8, block=8, n_polar=4, n_azim=8, L=1, p=3),
-                 P.batch_3d(n=4, block=2, quad=ou.Quadrature(ou.LS, 3, sn=4, L_max=0))):
+                 P.batch_3d(n=4, block=2, quad=ou.Quadrature(ou.LS, 3, sn=4, L_max=0)),
+                 P.headline_3d(n=4, block=4, n_polar=4, n_azim=8, L=1)):  # 3D Q2
         op = spec.operator(ou.FP64)
         assert op.sweep_mode == ou.SWEEP_LEVELS
         with pytest.raises(ou.OuError) as e:
