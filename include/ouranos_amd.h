@@ -379,13 +379,16 @@ typedef struct {
   int waves;
   int dir_groups;
   double ms; /* mean event time of one batched application */
+  int mode;  /* OU_SWEEP_LEVELS (the chunk sweep's shape) or OU_SWEEP_TILES */
 } ou_tune_entry;
 
 /* A batch of n_plans distinct plans sharing dimension, element order, moment
  * count and precision (meshes, materials, quadratures and sources may all
- * differ).  waves / dir_groups <= 0: the autotuner picks the shape, from its
- * cache (OURANOS_TUNE_CACHE names a file that persists it) or by timing every
- * supported shape on this batch.  The batch sets every member's sweep shape;
+ * differ).  waves / dir_groups <= 0: the autotuner picks the sweep mode and
+ * its shape, from its cache (OURANOS_TUNE_CACHE names a file that persists
+ * it) or by timing every supported shape of the chunk sweep and, when every
+ * member has one, of the tile sweep on this batch; waves / dir_groups > 0 set
+ * the chunk sweep's shape.  The batch sets every member's sweep shape;
  * members keep their own state (scattering source, traces, partials), so
  * batched and single calls on one plan must be ordered by the caller, as
  * must batched calls on one batch issued to different streams.  The plans
@@ -405,10 +408,16 @@ int ou_batch_set_graph(ou_batch *batch, int use_graph);
  * bit-for-bit the same. */
 int ou_batch_set_streams(ou_batch *batch, int n_streams);
 /* OU_SWEEP_PERSISTENT sweeps every member's levels in one persistent launch
- * (inside the captured graph); OU_SWEEP_LEVELS launches the merged levels.
- * Bit-for-bit the same results.  A watchdog trip (the first member's wait
- * limit) makes the next synchronising batched call return OU_ERR_TIMEOUT. */
+ * (inside the captured graph); OU_SWEEP_LEVELS launches the merged levels;
+ * OU_SWEEP_TILES launches the members' merged tile levels (every member needs
+ * a tile sweep, see ou_set_sweep_mode; OU_ERR_UNSUPPORTED otherwise).  At
+ * equal direction groups the results are bit-for-bit the same, and equal each
+ * member's own ou_apply in the same mode and shape.  ou_batch_set_shape and
+ * ou_batch_info set and read the shape of the mode in use.  A watchdog trip
+ * (the first member's wait limit) makes the next synchronising batched call
+ * return OU_ERR_TIMEOUT. */
 int ou_batch_set_sweep_mode(ou_batch *batch, int mode);
+int ou_batch_sweep_mode(const ou_batch *batch, int *mode);
 
 /* Apply of every member: d_y[i] = d_x[i] - D L^-1 M S d_x[i] (device vectors;
  * d_x / d_y are host arrays of n_plans device pointers; all d_y distinct and

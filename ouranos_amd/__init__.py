@@ -79,7 +79,8 @@ class SolveInfo(ctypes.Structure):
 
 
 class TuneEntry(ctypes.Structure):
-    _fields_ = [("waves", ctypes.c_int), ("dir_groups", ctypes.c_int), ("ms", ctypes.c_double)]
+    _fields_ = [("waves", ctypes.c_int), ("dir_groups", ctypes.c_int), ("ms", ctypes.c_double),
+                ("mode", ctypes.c_int)]
 
 
 class Timing(ctypes.Structure):
@@ -147,6 +148,7 @@ def lib():
         L.ou_sweep_shape.argtypes = [_vp, _ip, _ip]
         L.ou_set_sweep_streams.argtypes = [_vp, ctypes.c_int]
         L.ou_set_sweep_mode.argtypes = [_vp, ctypes.c_int]
+        L.ou_batch_sweep_mode.argtypes = [_vp, _ip]
         L.ou_sweep_mode.argtypes = [_vp, _ip]
         L.ou_set_sweep_wait_limit.argtypes = [_vp, ctypes.c_double]
         L.ou_sweep_status.argtypes = [_vp]
@@ -616,8 +618,16 @@ class Batch:
         _check(lib().ou_batch_set_streams(self._b, int(n)))
 
     def set_sweep_mode(self, mode):
-        """SWEEP_PERSISTENT: every member's levels in one persistent launch."""
+        """SWEEP_LEVELS (merged level launches), SWEEP_PERSISTENT (every
+        member's levels in one persistent launch) or SWEEP_TILES (merged tile
+        levels, when every member has a tile sweep)."""
         _check(lib().ou_batch_set_sweep_mode(self._b, int(mode)))
+
+    @property
+    def sweep_mode(self):
+        m = ctypes.c_int()
+        _check(lib().ou_batch_sweep_mode(self._b, ctypes.byref(m)))
+        return m.value
 
     def apply_device(self, xs, ys, stream=None):
         _check(lib().ou_batch_apply_device(self._b, _ptr_table(xs), _ptr_table(ys), stream))
@@ -679,7 +689,7 @@ class Batch:
         entries = (TuneEntry * 64)()
         n = ctypes.c_int()
         _check(lib().ou_batch_tune(self._b, int(reps), entries, 64, ctypes.byref(n)))
-        return [{"waves": e.waves, "dir_groups": e.dir_groups, "ms": e.ms}
+        return [{"waves": e.waves, "dir_groups": e.dir_groups, "ms": e.ms, "mode": e.mode}
                 for e in entries[:min(n.value, 64)]]
 
     def time_apply(self, xs, ys, warmup, steps):

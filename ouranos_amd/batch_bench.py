@@ -79,14 +79,17 @@ def main(argv=None):
     fixed = tuple(int(v) for v in a.shape.split(",")) if a.shape else (4, 1)
     batch = ou.Batch(ops, waves=fixed[0], dir_groups=fixed[1])
     log(f"[batch] batch built in {time.perf_counter() - t0:.1f} s: {batch.info()}")
+    mode_name = {ou.SWEEP_LEVELS: "levels", ou.SWEEP_PERSISTENT: "persistent",
+                 ou.SWEEP_TILES: "tiles"}
     table = None
     if not a.shape:
         table = batch.tune(reps=a.tune_reps)
         for e in sorted(table, key=lambda e: e["ms"]):
-            log(f"[batch]   waves={e['waves']:2d} dir_groups={e['dir_groups']}  "
-                f"{e['ms']:.2f} ms")
+            log(f"[batch]   {mode_name[e['mode']]:6s} waves={e['waves']:2d} "
+                f"dir_groups={e['dir_groups']}  {e['ms']:.2f} ms")
     info = batch.info()
-    log(f"[batch] shape: waves={info['waves']} dir_groups={info['dir_groups']} "
+    mode = batch.sweep_mode
+    log(f"[batch] {mode_name[mode]}, shape: waves={info['waves']} dir_groups={info['dir_groups']} "
         f"({'fixed' if a.shape else 'autotuned'})")
 
     graph_ms = batch.time_apply(xs, ys, a.warmup, a.steps)
@@ -100,13 +103,27 @@ def main(argv=None):
     batch.set_streams(2)
     log(f"[batch] graph replay, one launch per level (no octant-half side stream): "
         f"{one_stream_ms:.2f} ms")
-    # every member's levels in one persistent launch inside the graph
+    # the level launches at their best shape, and every member's levels in one
+    # persistent launch inside the graph at that shape
+    lev = [e for e in (table or []) if e["mode"] == ou.SWEEP_LEVELS]
+    lev_shape = min(lev, key=lambda e: e["ms"]) if lev else None
+    levels_ms = persistent_ms = None
+    if mode != ou.SWEEP_LEVELS and lev_shape:
+        batch.set_sweep_mode(ou.SWEEP_LEVELS)
+        batch.set_shape(lev_shape["waves"], lev_shape["dir_groups"])
+        levels_ms = batch.time_apply(xs, ys, a.warmup, a.steps)
+        log(f"[batch] graph replay, merged level launches at ({lev_shape['waves']},"
+            f"{lev_shape['dir_groups']}): {levels_ms:.2f} ms")
     batch.set_sweep_mode(ou.SWEEP_PERSISTENT)
     persistent_ms = batch.time_apply(xs, ys, a.warmup, a.steps)
-    batch.set_sweep_mode(ou.SWEEP_LEVELS)
     log(f"[batch] graph replay, one persistent sweep launch: {persistent_ms:.2f} ms")
+    batch.set_sweep_mode(mode)
+    batch.set_shape(info["waves"], info["dir_groups"])
 
-    # bit-for-bit spot check of two members against their own Apply
+    # bit-for-bit spot check of two members against their own Apply in the
+    # batch's mode and shape
+    for op in ops:
+        op.sweep_mode = ou.SWEEP_TILES if mode == ou.SWEEP_TILES else ou.SWEEP_LEVELS
     batch.apply_device(xs, ys)
     ou.lib().ou_device_synchronize()
     order = np.argsort([s.n_cells for s in specs])
@@ -157,7 +174,9 @@ def main(argv=None):
         "sizes": list(sizes), "block": a.block, "cells": cells, "directions": specs[0].quad.n_dir,
         "precision": "fp32 sweep, fp64 moments",
         "shape": {"waves": info["waves"], "dir_groups": info["dir_groups"]},
+        "mode": mode_name[mode],
         "ms_graph_one_stream": one_stream_ms,
+        "ms_graph_levels_best": levels_ms,
         "ms_graph_persistent": persistent_ms,
         "levels_per_batch": info["n_levels"], "chunks_per_batch": info["n_chunks"],
         "ms_graph": graph_ms, "ms_eager": eager_ms, "ms_sequential": seq_ms,

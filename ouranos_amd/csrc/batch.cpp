@@ -20,12 +20,16 @@ namespace ou {
 namespace {
 
 // ---------------------------------------------------------------------------
-// Tuning cache: key -> (waves, direction groups).  Kept for the process; when
-// OURANOS_TUNE_CACHE names a file, entries are read from it on first use and
-// every new result is appended ("<key> <waves> <groups>" per line).
+// Tuning cache: key -> (sweep mode, waves, direction groups).  Kept for the
+// process; when OURANOS_TUNE_CACHE names a file, entries are read from it on
+// first use and every new result is appended ("<key> | <mode> <waves>
+// <groups>" per line).
 // ---------------------------------------------------------------------------
+struct TuneChoice {
+  int mode = OU_SWEEP_LEVELS, waves = 0, groups = 0;
+};
 std::mutex g_tune_mu;
-std::map<std::string, std::pair<int, int>> g_tune_cache;
+std::map<std::string, TuneChoice> g_tune_cache;
 bool g_tune_loaded = false;
 
 void load_tune_cache_locked() {
@@ -36,30 +40,31 @@ void load_tune_cache_locked() {
   std::ifstream in(path);
   std::string line;
   while (std::getline(in, line)) {
-    const size_t q = line.rfind(' '), p = q == std::string::npos ? q : line.rfind(' ', q - 1);
-    if (p == std::string::npos || p == 0) continue;
-    const int w = std::atoi(line.c_str() + p + 1), g = std::atoi(line.c_str() + q + 1);
-    if (w > 0 && g > 0) g_tune_cache[line.substr(0, p)] = {w, g};
+    const size_t bar = line.rfind(" | ");
+    if (bar == std::string::npos || bar == 0) continue;
+    std::istringstream rest(line.substr(bar + 3));
+    TuneChoice c;
+    if (!(rest >> c.mode >> c.waves >> c.groups)) continue;
+    if (c.waves > 0 && c.groups > 0) g_tune_cache[line.substr(0, bar)] = c;
   }
 }
 
-bool tune_lookup(const std::string &key, int &w, int &g) {
+bool tune_lookup(const std::string &key, TuneChoice &c) {
   std::lock_guard<std::mutex> lock(g_tune_mu);
   load_tune_cache_locked();
   auto it = g_tune_cache.find(key);
   if (it == g_tune_cache.end()) return false;
-  w = it->second.first;
-  g = it->second.second;
+  c = it->second;
   return true;
 }
 
-void tune_store(const std::string &key, int w, int g) {
+void tune_store(const std::string &key, const TuneChoice &c) {
   std::lock_guard<std::mutex> lock(g_tune_mu);
   load_tune_cache_locked();
-  g_tune_cache[key] = {w, g};
+  g_tune_cache[key] = c;
   if (const char *path = std::getenv("OURANOS_TUNE_CACHE")) {
     std::ofstream out(path, std::ios::app);
-    out << key << ' ' << w << ' ' << g << '\n';
+    out << key << " | " << c.mode << ' ' << c.waves << ' ' << c.groups << '\n';
   }
 }
 
@@ -110,11 +115,13 @@ Batch::Batch(std::vector<RadiativeTransfer *> members, int waves, int dir_groups
     if (waves > 0 && dir_groups > 0) {
       set_shape(waves, dir_groups);
     } else {
-      int w = 0, g = 0;
-      if (tune_lookup(tune_key(), w, g))
-        set_shape(w, g);
-      else
+      TuneChoice c;
+      if (tune_lookup(tune_key(), c) && (c.mode != OU_SWEEP_TILES || bl_.tiles_ok)) {
+        set_sweep_mode(c.mode);
+        set_shape(c.waves, c.groups);
+      } else {
         tune(3);
+      }
     }
   } catch (...) {
     release();
@@ -132,6 +139,7 @@ void Batch::release() {
   d_dep_off_ = d_dep_list_ = d_error_ = nullptr;
   d_done_ = nullptr;
   for (void *p : {static_cast<void *>(d_chunks_), static_cast<void *>(d_chunk_view_),
+                  static_cast<void *>(d_tiles_), static_cast<void *>(d_tile_view_),
                   static_cast<void *>(d_views_), static_cast<void *>(d_members_),
                   static_cast<void *>(d_io_), static_cast<void *>(d_si_work_),
                   static_cast<void *>(d_si_partial_), static_cast<void *>(d_si_norms_),
@@ -147,6 +155,8 @@ void Batch::release() {
   h_si_norms_ = nullptr;
   d_chunks_ = nullptr;
   d_chunk_view_ = nullptr;
+  d_tiles_ = nullptr;
+  d_tile_view_ = nullptr;
   d_views_ = nullptr;
   d_members_ = nullptr;
   d_io_ = nullptr;
@@ -252,6 +262,43 @@ void Batch::build_schedule() {
   }
   bl_.waves = p0.waves;
   bl_.dir_groups = p0.dir_groups;
+
+  // The merged tile schedule (OU_SWEEP_TILES): tile level l of the batch
+  // holds tile level l of every member.
+  bl_.tiles_ok = true;
+  for (const RadiativeTransfer *m : members_) bl_.tiles_ok = bl_.tiles_ok && m->plan().tiles_ok;
+  if (bl_.tiles_ok) {
+    size_t n_tl = 0;
+    std::vector<std::vector<TileItem>> th(members_.size());
+    for (size_t u = 0; u < members_.size(); ++u) {
+      const Plan &pl = members_[u]->plan();
+      n_tl = std::max(n_tl, pl.tile_level_off.size() - 1);
+      th[u].resize(size_t(pl.tile_level_off.back()));
+      if (!th[u].empty())
+        OU_HIP(hipMemcpy(th[u].data(), pl.d_tiles, th[u].size() * sizeof(TileItem),
+                         hipMemcpyDeviceToHost));
+    }
+    std::vector<TileItem> tiles;
+    std::vector<int> tview;
+    bl_.tile_level_off.assign(1, 0);
+    for (size_t l = 0; l < n_tl; ++l) {
+      for (size_t u = 0; u < members_.size(); ++u) {
+        const Plan &pl = members_[u]->plan();
+        if (l + 1 >= pl.tile_level_off.size()) continue;
+        for (int i = pl.tile_level_off[l]; i < pl.tile_level_off[l + 1]; ++i) {
+          tiles.push_back(th[u][size_t(i)]);
+          tview.push_back(int(u));
+        }
+      }
+      bl_.tile_level_off.push_back(int(tiles.size()));
+    }
+    d_tiles_ = dev_upload(tiles);
+    d_tile_view_ = dev_upload(tview);
+    bl_.tiles = d_tiles_;
+    bl_.tile_view = d_tile_view_;
+    bl_.tile_waves = p0.tile_waves;
+    bl_.tile_groups = p0.tile_groups;
+  }
 }
 
 void Batch::build_views() {
@@ -259,9 +306,12 @@ void Batch::build_views() {
   std::vector<SweepView> views(2 * n);
   std::vector<MemberView> mv(n);
   seen_partial_.assign(n, nullptr);
+  const bool tiles = bl_.sweep_mode == OU_SWEEP_TILES;
+  const int groups = tiles ? bl_.tile_groups : bl_.dir_groups;
   for (size_t u = 0; u < n; ++u) {
     const Plan &pl = members_[u]->plan();
     views[u] = pl.view();
+    views[u].dir_groups = groups;
     views[n + u] = views[u];
     views[n + u].S0 = pl.d_S0;
     MemberView &m = mv[u];
@@ -271,8 +321,8 @@ void Batch::build_views() {
     m.n_mom = pl.n_mom;
     m.L = pl.L;
     m.n_dir = pl.n_dir;
-    m.n_parts = pl.n_oct * pl.dir_groups;
-    m.groups = pl.dir_groups;
+    m.n_parts = pl.n_oct * groups;
+    m.groups = groups;
     m.oct_mask = pl.oct_mask;
     m.mat = pl.d_mat;
     m.sigma_s = pl.d_sigma_s;
@@ -311,20 +361,33 @@ void Batch::drop_graphs() {
 }
 
 void Batch::set_shape(int waves, int dir_groups) {
-  for (RadiativeTransfer *m : members_) set_sweep_shape(m->plan(), waves, dir_groups);
-  bl_.waves = waves;
-  bl_.dir_groups = dir_groups;
+  OU_HIP(hipStreamSynchronize(stream_));
+  if (bl_.sweep_mode == OU_SWEEP_TILES) {
+    for (RadiativeTransfer *m : members_) set_tile_shape(m->plan(), waves, dir_groups);
+    bl_.tile_waves = waves;
+    bl_.tile_groups = dir_groups;
+  } else {
+    for (RadiativeTransfer *m : members_) set_sweep_shape(m->plan(), waves, dir_groups);
+    bl_.waves = waves;
+    bl_.dir_groups = dir_groups;
+  }
   batch_prepare(bl_);
   drop_graphs();
   build_views();
 }
 
 void Batch::set_sweep_mode(int mode) {
-  if (mode != OU_SWEEP_LEVELS && mode != OU_SWEEP_PERSISTENT)
-    fail(OU_ERR_ARG, "sweep mode must be OU_SWEEP_LEVELS or OU_SWEEP_PERSISTENT");
+  if (mode != OU_SWEEP_LEVELS && mode != OU_SWEEP_PERSISTENT && mode != OU_SWEEP_TILES)
+    fail(OU_ERR_ARG,
+         "sweep mode must be OU_SWEEP_LEVELS, OU_SWEEP_PERSISTENT or OU_SWEEP_TILES");
+  if (mode == OU_SWEEP_TILES && !bl_.tiles_ok)
+    fail(OU_ERR_UNSUPPORTED, "a member of this batch has no tile sweep");
   OU_HIP(hipStreamSynchronize(stream_));
+  const bool was_tiles = bl_.sweep_mode == OU_SWEEP_TILES;
   bl_.sweep_mode = mode;
   drop_graphs();
+  // the views carry the mode's direction groups
+  if (was_tiles != (mode == OU_SWEEP_TILES)) set_shape(waves(), dir_groups());
 }
 
 void Batch::check_watchdog(hipStream_t s) {
@@ -343,13 +406,16 @@ void Batch::set_streams(int n) {
 // shape and rewrite the tables.
 void Batch::ensure_current() {
   bool stale = d_views_ == nullptr;
+  const bool tiles = bl_.sweep_mode == OU_SWEEP_TILES;
   for (size_t u = 0; u < members_.size(); ++u) {
     const Plan &pl = members_[u]->plan();
-    if (pl.waves != bl_.waves || pl.dir_groups != bl_.dir_groups ||
-        (u < seen_partial_.size() && pl.d_partial != seen_partial_[u]))
+    const bool shape_differs = tiles ? (pl.tile_waves != bl_.tile_waves ||
+                                        pl.tile_groups != bl_.tile_groups)
+                                     : (pl.waves != bl_.waves || pl.dir_groups != bl_.dir_groups);
+    if (shape_differs || (u < seen_partial_.size() && pl.d_partial != seen_partial_[u]))
       stale = true;
   }
-  if (stale) set_shape(bl_.waves, bl_.dir_groups);
+  if (stale) set_shape(waves(), dir_groups());
 }
 
 void Batch::upload_io(const double *const *xs, double *const *ys, hipStream_t s) {
@@ -544,19 +610,29 @@ std::vector<TuneEntry> Batch::tune(int reps) {
     vec_fill(max_vec, 1.0, x, stream_);
     OU_HIP(hipEventCreate(&e0));
     OU_HIP(hipEventCreate(&e1));
+    auto time_shape = [&](int mode, int w, int g) {
+      set_shape(w, g);
+      apply(xs.data(), ys.data(), stream_);  // warm-up, captures the graph
+      OU_HIP(hipStreamSynchronize(stream_));
+      OU_HIP(hipEventRecord(e0, stream_));
+      for (int r = 0; r < reps; ++r) apply(xs.data(), ys.data(), stream_);
+      OU_HIP(hipEventRecord(e1, stream_));
+      OU_HIP(hipEventSynchronize(e1));
+      float ms = 0.0f;
+      OU_HIP(hipEventElapsedTime(&ms, e0, e1));
+      out.push_back({w, g, double(ms) / reps, mode});
+    };
+    // the chunk sweep's (waves, groups), then the tile sweep's
+    set_sweep_mode(OU_SWEEP_LEVELS);
     for (int w : {2, 4, 6, 8, 12}) {
       if (!sweep_supported(p0.dim, p0.p, p0.mm, p0.precision, w, p0.max_lds)) continue;
-      for (int g = min_g; g <= max_g; ++g) {
-        set_shape(w, g);
-        apply(xs.data(), ys.data(), stream_);  // warm-up, captures the graph
-        OU_HIP(hipStreamSynchronize(stream_));
-        OU_HIP(hipEventRecord(e0, stream_));
-        for (int r = 0; r < reps; ++r) apply(xs.data(), ys.data(), stream_);
-        OU_HIP(hipEventRecord(e1, stream_));
-        OU_HIP(hipEventSynchronize(e1));
-        float ms = 0.0f;
-        OU_HIP(hipEventElapsedTime(&ms, e0, e1));
-        out.push_back({w, g, double(ms) / reps});
+      for (int g = min_g; g <= max_g; ++g) time_shape(OU_SWEEP_LEVELS, w, g);
+    }
+    if (bl_.tiles_ok) {
+      set_sweep_mode(OU_SWEEP_TILES);
+      for (int w : {1, 2, 4}) {
+        if (!tile_sweep_supported(p0.dim, p0.p, p0.mm, p0.precision, w, p0.max_lds)) continue;
+        for (int g = min_g; g <= max_g; ++g) time_shape(OU_SWEEP_TILES, w, g);
       }
     }
   } catch (...) {
@@ -576,8 +652,9 @@ std::vector<TuneEntry> Batch::tune(int reps) {
   const TuneEntry best =
       *std::min_element(out.begin(), out.end(),
                         [](const TuneEntry &a, const TuneEntry &b) { return a.ms < b.ms; });
+  set_sweep_mode(best.mode);
   set_shape(best.waves, best.dir_groups);
-  tune_store(tune_key(), best.waves, best.dir_groups);
+  tune_store(tune_key(), TuneChoice{best.mode, best.waves, best.dir_groups});
   return out;
 }
 

@@ -23,6 +23,7 @@ namespace ou {
 struct TuneEntry {
   int waves = 0, dir_groups = 0;
   double ms = 0.0;  // mean time of one batched Apply
+  int mode = OU_SWEEP_LEVELS;  // the sweep mode the shape belongs to
 };
 
 class Batch {
@@ -35,8 +36,11 @@ class Batch {
   Batch &operator=(const Batch &) = delete;
 
   int size() const { return int(members_.size()); }
-  int waves() const { return bl_.waves; }
-  int dir_groups() const { return bl_.dir_groups; }
+  // the shape of the sweep mode in use
+  int waves() const { return bl_.sweep_mode == OU_SWEEP_TILES ? bl_.tile_waves : bl_.waves; }
+  int dir_groups() const {
+    return bl_.sweep_mode == OU_SWEEP_TILES ? bl_.tile_groups : bl_.dir_groups;
+  }
   int n_levels() const { return int(bl_.level_off.size()) - 1; }
   long n_chunks() const { return bl_.level_off.back(); }
   hipStream_t stream() const { return stream_; }
@@ -58,8 +62,10 @@ class Batch {
   int solve_si(double *const *d_phi, double tol, int max_it, int *iterations, double *change,
                bool verbose);
 
+  // The shape of the sweep mode in use (OU_SWEEP_TILES: the tile sweep's).
   void set_shape(int waves, int dir_groups);
-  // OU_SWEEP_LEVELS or OU_SWEEP_PERSISTENT (one persistent launch).
+  // OU_SWEEP_LEVELS, OU_SWEEP_PERSISTENT (one persistent launch) or
+  // OU_SWEEP_TILES (merged tile levels; every member needs a tile sweep).
   void set_sweep_mode(int mode);
   int sweep_mode() const { return bl_.sweep_mode; }
   // OU_ERR_TIMEOUT if a persistent-sweep wait gave up since the last check
@@ -71,8 +77,9 @@ class Batch {
   void set_streams(int n);
   int streams() const { return bl_.side ? 2 : 1; }
   bool use_graph() const { return use_graph_; }
-  // Times every supported (waves, direction groups) shape on this batch,
-  // keeps the fastest and records it in the tuning cache.
+  // Times every supported (waves, direction groups) shape of the chunk sweep
+  // and, where every member has one, of the tile sweep on this batch, keeps
+  // the fastest (mode and shape) and records it in the tuning cache.
   std::vector<TuneEntry> tune(int reps);
   // The cache key of this batch's shape class.
   std::string tune_key() const;
@@ -97,6 +104,8 @@ class Batch {
   int *d_dep_off_ = nullptr, *d_dep_list_ = nullptr, *d_error_ = nullptr;
   unsigned *d_done_ = nullptr;  // persistent launch: completion flags, then the ticket
   int *d_chunk_view_ = nullptr;
+  TileItem *d_tiles_ = nullptr;  // merged tile levels
+  int *d_tile_view_ = nullptr;
   SweepView *d_views_ = nullptr;     // [2][members]: plain, with source
   MemberView *d_members_ = nullptr;  // [members]
   const double **d_io_ = nullptr;    // [2][members]: x pointers, then y pointers

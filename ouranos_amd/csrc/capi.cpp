@@ -666,6 +666,14 @@ int ou_batch_set_sweep_mode(ou_batch *batch, int mode) {
   });
 }
 
+int ou_batch_sweep_mode(const ou_batch *batch, int *mode) {
+  return guarded([&] {
+    if (!mode) ou::fail(OU_ERR_ARG, "null output pointer");
+    *mode = batch_of(batch).sweep_mode();
+    return int(OU_OK);
+  });
+}
+
 int ou_batch_create(ou_plan *const *plans, int n_plans, int waves, int dir_groups,
                     ou_batch **batch) {
   return guarded([&] {
@@ -755,7 +763,8 @@ int ou_batch_tune(ou_batch *batch, int reps, ou_tune_entry *entries, int max_ent
     const std::vector<ou::TuneEntry> t = batch_of(batch).tune(reps);
     const int n = std::min<int>(int(t.size()), std::max(0, max_entries));
     for (int i = 0; i < n && entries; ++i)
-      entries[i] = ou_tune_entry{t[size_t(i)].waves, t[size_t(i)].dir_groups, t[size_t(i)].ms};
+      entries[i] = ou_tune_entry{t[size_t(i)].waves, t[size_t(i)].dir_groups, t[size_t(i)].ms,
+                                 t[size_t(i)].mode};
     if (n_entries) *n_entries = int(t.size());
     return int(OU_OK);
   });

@@ -238,6 +238,13 @@ struct BatchLaunch {
   int sweep_mode = OU_SWEEP_LEVELS;
   PersistView persist = {};              // groups and n_items set by the launch
   int cus = 256;
+  // OU_SWEEP_TILES: tile level l of the batch is tiles [tile_level_off[l],
+  // tile_level_off[l+1]) of `tiles`, tile i of the member at views[tile_view[i]]
+  bool tiles_ok = false;                 // every member has a tile sweep
+  std::vector<int> tile_level_off;       // host
+  const TileItem *tiles = nullptr;
+  const int *tile_view = nullptr;
+  int tile_waves = 1, tile_groups = 1;
 };
 
 Plan *create_plan(const ou_problem_desc *desc);
@@ -281,6 +288,8 @@ void launch_epilogue(Plan &pl, const double *d_x, double *d_y, bool apply,
 // One launch per tile level (sweep_tile.hip).
 void launch_sweep_tiles(Plan &pl, const SweepView &v, hipStream_t s, hipEvent_t before,
                         hipEvent_t after);
+// The batch's merged tile levels (launch false only sets kernel attributes).
+void launch_batch_tiles(const BatchLaunch &b, const SweepView *views, hipStream_t s, bool launch);
 
 // Whole operations.
 void apply_device(Plan &pl, const double *d_x, double *d_y, hipStream_t s);

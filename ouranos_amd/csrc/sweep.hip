@@ -891,7 +891,9 @@ void batch_epilogue(const BatchLaunch &b, const double *const *d_xs, double *con
 // The merged levels of every member: one persistent launch or level launches.
 void batch_levels(const BatchLaunch &b, const LevelKernels *k, const SweepView *views,
                   hipStream_t s) {
-  if (b.sweep_mode == OU_SWEEP_PERSISTENT)
+  if (b.sweep_mode == OU_SWEEP_TILES)
+    launch_batch_tiles(b, views, s, true);
+  else if (b.sweep_mode == OU_SWEEP_PERSISTENT)
     k->batch_persistent(b, views, s, true);
   else
     k->batch(b, views, s, true);
@@ -903,6 +905,7 @@ void batch_prepare(const BatchLaunch &b) {
   const LevelKernels *k = batch_kernels(b);
   k->batch(b, nullptr, nullptr, false);
   k->batch_persistent(b, nullptr, nullptr, false);
+  if (b.tiles_ok) launch_batch_tiles(b, nullptr, nullptr, false);
 }
 
 void batch_apply(const BatchLaunch &b, const double *const *d_xs, double *const *d_ys,

@@ -384,7 +384,41 @@ __global__ __launch_bounds__(64 * W) void k_sweep_tile(SweepView v,
   sweep_tile_item<T, DIM, P, MM, W>(v, tiles[tile_begin + blockIdx.x], blockIdx.y);
 }
 
+// The batch's merged tile levels: tile i belongs to the member whose view is
+// views[tile_view[i]]; members whose flag is cleared are skipped.
+template <typename T, int DIM, int P, int MM, int W>
+__global__ __launch_bounds__(64 * W) void k_sweep_tile_batch(const SweepView *__restrict__ views,
+                                                             const TileItem *__restrict__ tiles,
+                                                             const int *__restrict__ tile_view,
+                                                             int tile_begin,
+                                                             const int *__restrict__ active) {
+  const int i = tile_begin + int(blockIdx.x);
+  const int u = tile_view[i];
+  if (!active[u]) return;  // the whole workgroup, before any barrier
+  sweep_tile_item<T, DIM, P, MM, W>(views[u], tiles[i], blockIdx.y);
+}
+
 using TileLaunch = void (*)(const Plan &, const SweepView &, hipStream_t);
+using BatchTileLaunch = void (*)(const BatchLaunch &, const SweepView *, hipStream_t, bool);
+
+template <typename T, int DIM, int P, int MM, int W>
+void launch_batch_tiles_impl(const BatchLaunch &bl, const SweepView *views, hipStream_t s,
+                             bool launch) {
+  using C = TileCfg<T, DIM, P, MM, W>;
+  auto kern = k_sweep_tile_batch<T, DIM, P, MM, W>;
+  if (!launch) {
+    OU_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, int(C::lds_bytes)));
+    return;
+  }
+  const unsigned groups = unsigned(bl.tile_groups);
+  for (size_t l = 0; l + 1 < bl.tile_level_off.size(); ++l) {
+    const int b = bl.tile_level_off[l], e = bl.tile_level_off[l + 1];
+    if (e > b)
+      hipLaunchKernelGGL(kern, dim3(e - b, groups), dim3(64 * W), C::lds_bytes, s, views,
+                         bl.tiles, bl.tile_view, b, bl.active);
+  }
+}
 
 template <typename T, int DIM, int P, int MM, int W>
 void launch_tiles_impl(const Plan &pl, const SweepView &v, hipStream_t s) {
@@ -405,13 +439,26 @@ void launch_tiles_impl(const Plan &pl, const SweepView &v, hipStream_t s) {
   }
 }
 
+// The single-plan and batched launchers of one instantiation.
+struct TileKernels {
+  TileLaunch single;
+  BatchTileLaunch batch;
+};
+
+template <typename T, int DIM, int P, int MM, int W>
+const TileKernels *tile_kernels_of() {
+  static const TileKernels k{launch_tiles_impl<T, DIM, P, MM, W>,
+                             launch_batch_tiles_impl<T, DIM, P, MM, W>};
+  return &k;
+}
+
 template <typename T, int DIM, int P, int W>
-TileLaunch pick_tile_mm(int mm) {
+const TileKernels *pick_tile_mm(int mm) {
   switch (mm) {
-    case 1: return launch_tiles_impl<T, DIM, P, 1, W>;
-    case 4: return launch_tiles_impl<T, DIM, P, 4, W>;
-    case 10: if constexpr (DIM == 2) return launch_tiles_impl<T, DIM, P, 10, W>; break;
-    case 16: if constexpr (DIM == 2) return launch_tiles_impl<T, DIM, P, 16, W>; break;
+    case 1: return tile_kernels_of<T, DIM, P, 1, W>();
+    case 4: return tile_kernels_of<T, DIM, P, 4, W>();
+    case 10: if constexpr (DIM == 2) return tile_kernels_of<T, DIM, P, 10, W>(); break;
+    case 16: if constexpr (DIM == 2) return tile_kernels_of<T, DIM, P, 16, W>(); break;
   }
   return nullptr;
 }
@@ -419,7 +466,7 @@ TileLaunch pick_tile_mm(int mm) {
 // Instantiated: 2D Q1-Q2 (up to 16 moments) and 3D Q1 (up to 4 moments), at
 // 1, 2 and 4 waves.
 template <typename T>
-TileLaunch pick_tile(int dim, int p, int mm, int waves) {
+const TileKernels *pick_tile(int dim, int p, int mm, int waves) {
   if (dim == 2 && p == 1) {
     if (waves == 1) return pick_tile_mm<T, 2, 1, 1>(mm);
     if (waves == 2) return pick_tile_mm<T, 2, 1, 2>(mm);
@@ -463,14 +510,23 @@ int tile_waves_max(int dim, int p, int mm, int precision, int max_lds) {
 
 void launch_sweep_tiles(Plan &pl, const SweepView &v, hipStream_t s, hipEvent_t before,
                         hipEvent_t after) {
-  const TileLaunch k = pl.precision == OU_FP32
-                           ? pick_tile<float>(pl.dim, pl.p, pl.mm, pl.tile_waves)
-                           : pick_tile<double>(pl.dim, pl.p, pl.mm, pl.tile_waves);
+  const TileKernels *k = pl.precision == OU_FP32
+                             ? pick_tile<float>(pl.dim, pl.p, pl.mm, pl.tile_waves)
+                             : pick_tile<double>(pl.dim, pl.p, pl.mm, pl.tile_waves);
   if (!k || !pl.tiles_ok) fail(OU_ERR_UNSUPPORTED, "no tile sweep for this plan");
   if (before) OU_HIP(hipEventRecord(before, s));
-  k(pl, v, s);
+  k->single(pl, v, s);
   if (after) OU_HIP(hipEventRecord(after, s));
   OU_HIP(hipGetLastError());
+}
+
+void launch_batch_tiles(const BatchLaunch &b, const SweepView *views, hipStream_t s, bool launch) {
+  const TileKernels *k = b.precision == OU_FP32
+                             ? pick_tile<float>(b.dim, b.p, b.mm, b.tile_waves)
+                             : pick_tile<double>(b.dim, b.p, b.mm, b.tile_waves);
+  if (!k || !b.tiles_ok) fail(OU_ERR_UNSUPPORTED, "no tile sweep for this batch");
+  k->batch(b, views, s, launch);
+  if (launch) OU_HIP(hipGetLastError());
 }
 
 }  // namespace ou

@@ -147,11 +147,13 @@ def test_c3_batched_instances_full_size(oracle, precision):
           f"{info['n_levels']} merged levels", flush=True)
     xs = [P.seeded_vector(s.size, 500 + i) for i, s in enumerate(specs)]
     ys = batch.apply(xs)
-    # The batch runs the members' chunk schedule at its own shape; a member's
-    # own level launches at that shape run the same arithmetic.  (The members'
-    # default tile sweep sums the directions in other groups.)
+    # The batch sets every member's shape for its mode; a member's own Apply in
+    # that mode runs the same arithmetic (the persistent launch equals the
+    # level launches).
+    mode = batch.sweep_mode
+    print(f"[gpu] C3 batch mode {mode}", flush=True)
     for op in ops:
-        op.sweep_mode = ou.SWEEP_LEVELS
+        op.sweep_mode = ou.SWEEP_TILES if mode == ou.SWEEP_TILES else ou.SWEEP_LEVELS
     for i, (op, x, y) in enumerate(zip(ops, xs, ys)):
         assert np.array_equal(y, op.Apply(x)), i
     print("[gpu] C3 batch: every member equals its single-plan Apply bit-for-bit", flush=True)

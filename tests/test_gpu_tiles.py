@@ -231,3 +231,45 @@ def test_solvers_in_tile_mode_retrace_level_mode():
         b, ib = op.solve(solver=solver, tol=1e-10, max_it=300)
         assert ia["iterations"] == ib["iterations"] and np.array_equal(a, b), solver
     op.close()
+
+
+@pytest.mark.parametrize("precision,pname", PRECISIONS, ids=["fp64", "fp32"])
+def test_batch_tile_mode(precision, pname):
+    """A batch of uneven 3D Q1 members (4^3 material blocks) in the merged tile
+    levels: every member equals its own tile-mode Apply and the batch's level
+    launches at the same groups bit-for-bit, over graph replays; batched
+    source iteration retraces the level mode; the autotuner times tile
+    shapes too."""
+    specs = P.batch_instances(n_instances=3, n_rhs=2, sizes=(4, 8, 12), block=4, seed=5)
+    ops = [s.operator(precision) for s in specs]
+    assert all(op.sweep_mode == ou.SWEEP_TILES for op in ops)
+    batch = ou.Batch(ops, waves=4, dir_groups=2)
+    assert batch.sweep_mode == ou.SWEEP_LEVELS
+    xs = [P.seeded_vector(s.size, 700 + i) for i, s in enumerate(specs)]
+    lev = batch.apply(xs)
+    batch.set_sweep_mode(ou.SWEEP_TILES)
+    for w in (4, 2):
+        batch.set_shape(w, 2)
+        assert batch.info()["waves"] == w and batch.info()["dir_groups"] == 2
+        print(f"[gpu] batch tiles shape ({w},2) {pname}", flush=True)
+        for rep in range(2):  # eager capture, then a graph replay
+            ys = batch.apply(xs)
+            for i, (y, y_lev) in enumerate(zip(ys, lev)):
+                assert np.array_equal(y, y_lev), (w, rep, i)
+    for op, x, y in zip(ops, xs, lev):
+        assert op.sweep_shape[1] == 2
+        assert np.array_equal(op.Apply(x), y)
+    phis, info = batch.solve_si(tol=1e-8, max_it=200)
+    batch.set_sweep_mode(ou.SWEEP_LEVELS)
+    batch.set_shape(4, 2)
+    phis_lev, info_lev = batch.solve_si(tol=1e-8, max_it=200)
+    assert info["iterations"] == info_lev["iterations"]
+    for a, b in zip(phis, phis_lev):
+        assert np.array_equal(a, b)
+    table = batch.tune(reps=1)
+    assert {e["mode"] for e in table} == {ou.SWEEP_LEVELS, ou.SWEEP_TILES}
+    best = min(table, key=lambda e: e["ms"])
+    assert batch.sweep_mode == best["mode"]
+    assert (batch.info()["waves"], batch.info()["dir_groups"]) == (best["waves"], best["dir_groups"])
+    batch.close()
+
