@@ -230,7 +230,7 @@ def copy_bandwidth(n_bytes=1 << 30, reps=10):
 
 def config_leg(name, warmup, steps):
     """Another configuration's operator application in its plan's default sweep
-    mode (the 2D tile sweep for C1 and C2), FP32 and FP64: ms per application
+    mode (the tile sweep for C1, C2 and C4), FP32 and FP64: ms per application
     (HIP events) and the sweep kernels' share."""
     import ouranos_amd as ou
     from ouranos_amd import problems as P
@@ -344,7 +344,8 @@ def main():
     d_y.free()
     if not a.no_configs and spec.name == "headline_3d":
         line["configs"] = {"C1": config_leg("uniform_2d", a.warmup, a.steps),
-                           "C2": config_leg("hetero_2d", a.warmup, a.steps)}
+                           "C2": config_leg("hetero_2d", a.warmup, a.steps),
+                           "C4": config_leg("thick_3d", a.warmup, a.steps)}
     if not a.no_solve and spec.name == "headline_3d":
         try:
             line["time_to_tol"] = time_to_tolerance(spec, op, a.tol)

@@ -42,6 +42,8 @@ def main(argv=None):
     ap.add_argument("--skip-sequential", action="store_true")
     ap.add_argument("--shape", default=None,
                     help="waves,dir_groups: fixed shape instead of the autotuner")
+    ap.add_argument("--mode", default="levels", choices=["levels", "tiles"],
+                    help="sweep mode of a fixed --shape")
     ap.add_argument("--si-tol", type=float, default=0.0,
                     help="also solve every member by batched source iteration to this change")
     ap.add_argument("--si-max-it", type=int, default=1000)
@@ -77,7 +79,13 @@ def main(argv=None):
 
     t0 = time.perf_counter()
     fixed = tuple(int(v) for v in a.shape.split(",")) if a.shape else (4, 1)
-    batch = ou.Batch(ops, waves=fixed[0], dir_groups=fixed[1])
+    # a tile shape is set after the batch exists (its constructor's shape is the
+    # chunk sweep's)
+    batch = ou.Batch(ops, waves=fixed[0] if a.mode == "levels" else 4,
+                     dir_groups=fixed[1] if a.mode == "levels" else 1)
+    if a.shape and a.mode == "tiles":
+        batch.set_sweep_mode(ou.SWEEP_TILES)
+        batch.set_shape(*fixed)
     log(f"[batch] batch built in {time.perf_counter() - t0:.1f} s: {batch.info()}")
     mode_name = {ou.SWEEP_LEVELS: "levels", ou.SWEEP_PERSISTENT: "persistent",
                  ou.SWEEP_TILES: "tiles"}
