@@ -243,6 +243,7 @@ def config_leg(name, warmup, steps):
     out = {"workload": desc, "n_cells": list(spec.n), "n_dir": spec.quad.n_dir,
            "updates_per_sweep": spec.updates_per_sweep()}
     modes = {ou.SWEEP_LEVELS: "levels", ou.SWEEP_PERSISTENT: "persistent", ou.SWEEP_TILES: "tiles"}
+    ops = {}
     for prec, pname in ((ou.FP32, "fp32"), (ou.FP64, "fp64")):
         op = spec.operator(prec)
         if warmup > 0:
@@ -253,9 +254,41 @@ def config_leg(name, warmup, steps):
                       "sweep_kernel_ms": tm["ms_sweep_kernel"] / steps,
                       "launches_per_sweep": tm["n_sweep_launches"],
                       "updates_per_s": spec.updates_per_sweep() * steps / (tm["ms_total"] / 1000.0)}
-        op.close()
+        ops[pname] = op
     d_x.free()
     d_y.free()
+    if name == "uniform_2d":
+        # the configuration's own solve: FP64 source iteration to a 1e-10
+        # relative scalar-flux change (ouranos.cc:52-130)
+        t0 = time.perf_counter()
+        _, info = ops["fp64"].solve(solver=ou.SI, tol=1e-10, max_it=2000)
+        out["si_1e-10_fp64"] = {"seconds": time.perf_counter() - t0,
+                                "iterations": info["iterations"],
+                                "converged": bool(info["converged"])}
+    elif name == "hetero_2d":
+        # FP64 GMRES(30) against FGMRES(30) on the FP32 operator with FP64
+        # refinement, both to a 1e-8 FP64 relative residual (ouranos.cc:131-208)
+        nd = spec.n_dofs
+        t0 = time.perf_counter()
+        phi_p, info_p = ops["fp64"].solve(solver=ou.GMRES, tol=1e-8, max_it=5000, restart=30,
+                                          raise_on_fail=False)
+        wall_p = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        phi_r, info_r = ou.solve_refined(ops["fp64"], ops["fp32"], None, tol=1e-8, max_it=5000,
+                                         restart=30, inner_tol=3e-5, raise_on_fail=False)
+        wall_r = time.perf_counter() - t0
+        out["gmres_1e-8"] = {
+            "plain_fp64": {"seconds": wall_p, "iterations": info_p["iterations"],
+                           "rel_residual": info_p["rel_residual"],
+                           "converged": bool(info_p["converged"])},
+            "fp32_sweep_fp64_refined": {"seconds": wall_r, "iterations": info_r["iterations"],
+                                        "rel_residual": info_r["rel_residual"],
+                                        "converged": bool(info_r["converged"])},
+            "scalar_flux_rel_diff": float(np.linalg.norm(phi_r[:nd] - phi_p[:nd]) /
+                                          np.linalg.norm(phi_p[:nd])),
+            "ratio": wall_p / wall_r}
+    for op in ops.values():
+        op.close()
     return out
 
 
