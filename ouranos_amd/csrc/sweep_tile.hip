@@ -457,14 +457,13 @@ const TileKernels *pick_tile_mm(int mm) {
   switch (mm) {
     case 1: return tile_kernels_of<T, DIM, P, 1, W>();
     case 4: return tile_kernels_of<T, DIM, P, 4, W>();
-    case 10: if constexpr (DIM == 2) return tile_kernels_of<T, DIM, P, 10, W>(); break;
-    case 16: if constexpr (DIM == 2) return tile_kernels_of<T, DIM, P, 16, W>(); break;
+    case 10: return tile_kernels_of<T, DIM, P, 10, W>();
+    case 16: return tile_kernels_of<T, DIM, P, 16, W>();
   }
   return nullptr;
 }
 
-// Instantiated: 2D Q1-Q2 (up to 16 moments) and 3D Q1 (up to 4 moments), at
-// 1, 2 and 4 waves.
+// Instantiated: 2D Q1-Q2 and 3D Q1, up to 16 moments, at 1, 2 and 4 waves.
 template <typename T>
 const TileKernels *pick_tile(int dim, int p, int mm, int waves) {
   if (dim == 2 && p == 1) {

@@ -51,6 +51,8 @@ def _cases():
         "batch_3d_q1_s12": lambda: P.batch_3d(n=12, block=4),
         "batch_3d_10_one_block": lambda: P.batch_3d(n=10, block=16),
         "headline_3d_q1_L1": lambda: P.headline_3d(n=8, block=4, n_polar=4, n_azim=8, L=1, p=1),
+        # 16 moments (L=3) in 3D
+        "headline_3d_q1_L3": lambda: P.headline_3d(n=8, block=4, n_polar=4, n_azim=8, L=3, p=1),
     }
 
 
@@ -273,3 +275,30 @@ def test_batch_tile_mode(precision, pname):
     assert (batch.info()["waves"], batch.info()["dir_groups"]) == (best["waves"], best["dir_groups"])
     batch.close()
 
+
+
+@pytest.mark.parametrize("bc", ["isotropic", "most_normal", "mixed"])
+def test_tile_source_sweep_with_inflow_matches_oracle(oracle, bc):
+    """The right-hand-side sweep in the tile sweep, 2D and 3D: volumetric
+    source and boundary inflow on the tiles' upwind faces
+    (RadiativeTransfer.cc:914-925, 957-975), FP64 at 1e-12."""
+    for spec in (P.hetero_2d(n=16, block=8, n_polar=4, n_azim=8, L=1),
+                 P.thick_3d(n=8, block=4, quad=ou.Quadrature(ou.LS, 3, sn=4, L_max=0))):
+        nf = 2 * spec.dim
+        if bc == "isotropic":
+            spec.bc, spec.inc = [1] * nf, [0.5 + f for f in range(nf)]
+        elif bc == "most_normal":
+            spec.bc, spec.inc = [2] * nf, [1.0 + f for f in range(nf)]
+        else:
+            spec.bc, spec.inc = [0, 1, 2, 0, 1, 2][:nf], [0.0, 2.0, 3.0, 0.0, 1.0, 0.5][:nf]
+        op = spec.operator(ou.FP64)
+        assert op.sweep_mode == ou.SWEEP_TILES
+        pb = oracle_problem(oracle, spec)
+        x = P.seeded_vector(spec.size, 5)
+        op.compute_scattering_source(x)
+        y0 = P.seeded_vector(spec.size, 6)
+        y = op.sweep(y0.copy(), with_source=True)
+        S = pb.scatter_source(x)
+        assert rel_err(y, y0 + pb.sweep(S, with_source=True)) < FP64_TOL, (spec.dim, bc)
+        assert rel_err(op.sweep(None, with_source=False), pb.sweep(S)) < FP64_TOL
+        op.close()
