@@ -348,15 +348,20 @@ int ou_sweep_streams(const ou_plan *plan, int *n_streams);
  * items in level order and start one as soon as the chunks upwind of it in
  * its own octant and group are done, the dataflow of the reference's
  * scheduler (RadiativeTransfer.cc:605-778, Task.cc:52-62) with the chunk as
- * the unit of readiness.  OU_SWEEP_TILES (2D, Q1-Q2, one material per 8x8
- * tile of cells): one launch per tile level; a workgroup sweeps a tile's 15
- * cell levels with the traces in registers (OU_ERR_UNSUPPORTED for other
- * plans).  All modes run the same per-cell arithmetic in the same order: at
- * equal direction groups the results are bit-for-bit the same.
+ * the unit of readiness.  OU_SWEEP_TILES (2D Q1-Q2 with one material per 8x8
+ * tile of cells, 3D Q1 with one material per 4x4x4 tile): one launch per tile
+ * level; a workgroup sweeps a tile's cell levels with the traces in registers
+ * (OU_ERR_UNSUPPORTED for other plans).  These modes run the same per-cell
+ * arithmetic in the same order: at equal direction groups the results are
+ * bit-for-bit the same.  3D Q2 plans in FP32 with at most 4 moments and one
+ * material per 4x4x4 tile also accept OU_SWEEP_TILES: four directions per
+ * wave are solved together on the matrix cores, with results within FP32
+ * rounding of the level launches (not the same bits), 3 or 4 waves.
  * ou_set_sweep_shape / ou_sweep_shape set and read the shape of the mode in
- * use (for OU_SWEEP_TILES: waves of 8 directions each, 1, 2 or 4, and
- * direction groups per quadrant).  A new plan starts in OU_SWEEP_TILES where
- * it has one and in OU_SWEEP_LEVELS otherwise. */
+ * use (for OU_SWEEP_TILES: waves of 8 directions each in 2D and 4 in 3D,
+ * and direction groups per octant).  A new plan starts in OU_SWEEP_TILES
+ * where it has a tile sweep, except 3D Q2 plans, whose level launches are
+ * faster, and in OU_SWEEP_LEVELS otherwise. */
 int ou_set_sweep_mode(ou_plan *plan, int mode);
 int ou_sweep_mode(const ou_plan *plan, int *mode);
 

@@ -535,9 +535,10 @@ class RadiativeTransfer:
     @property
     def sweep_mode(self):
         """SWEEP_LEVELS (one launch per wavefront level), SWEEP_PERSISTENT
-        (one persistent launch per sweep) or SWEEP_TILES (2D Q1-Q2 with one
-        material per 8x8 tile: one launch per tile level).  sweep_shape reads
-        and sets the shape of the mode in use."""
+        (one persistent launch per sweep) or SWEEP_TILES (one launch per tile
+        level: 2D Q1-Q2 with one material per 8x8 tile, 3D Q1 with one per
+        4x4x4 tile, and, opt-in, 3D Q2 in FP32 with at most 4 moments).
+        sweep_shape reads and sets the shape of the mode in use."""
         m = ctypes.c_int()
         _check(lib().ou_sweep_mode(self._plan, ctypes.byref(m)))
         return m.value

@@ -1,4 +1,4 @@
-// The 2D tile sweep (OU_SWEEP_TILES): one launch per tile level instead of
+// The tile sweep (OU_SWEEP_TILES): one launch per tile level instead of
 // one per cell level.
 //
 // A workgroup sweeps one 8 x 8 tile of one quadrant (all of one material) for
@@ -13,14 +13,19 @@
 // launches of 15 register-resident levels instead of 511 launches that each
 // pass every trace through memory.
 //
+// The 3D Q1 tiles are 4 x 4 x 4 cells: lane = 16 x direction + 4 b + a,
+// cell (a, b, t - a - b) at tile level t.
+//
 // The per-cell arithmetic is the level sweep's own, in the same order (M2D
 // rows of the staged source, upwind terms x then y, the (material, class)
 // inverse, FP64 D2M accumulation in slot order per direction group), so at
 // the same direction groups both modes give the same bits.
 // (RadiativeTransfer.cc:878-1012 per cell; the level sweep in sweep.hip.)
+// The 3D Q2 tiles solve on the matrix cores instead (sweep_tile_q2.inc).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "plan.hpp"
 #include "sweep_common.hpp"
@@ -398,6 +403,8 @@ __global__ __launch_bounds__(64 * W) void k_sweep_tile_batch(const SweepView *__
   sweep_tile_item<T, DIM, P, MM, W>(views[u], tiles[i], blockIdx.y);
 }
 
+#include "sweep_tile_q2.inc"
+
 using TileLaunch = void (*)(const Plan &, const SweepView &, hipStream_t);
 using BatchTileLaunch = void (*)(const BatchLaunch &, const SweepView *, hipStream_t, bool);
 
@@ -439,11 +446,66 @@ void launch_tiles_impl(const Plan &pl, const SweepView &v, hipStream_t s) {
   }
 }
 
+template <int MM, int W>
+void launch_q2_batch_tiles_impl(const BatchLaunch &bl, const SweepView *views, hipStream_t s,
+                                bool launch) {
+  using C = Q2TileCfg<MM, W>;
+  auto kern = k_sweep_tile_q2_batch<MM, W>;
+  if (!launch) {
+    OU_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, int(C::lds_bytes)));
+    return;
+  }
+  const unsigned groups = unsigned(bl.tile_groups);
+  for (size_t l = 0; l + 1 < bl.tile_level_off.size(); ++l) {
+    const int b = bl.tile_level_off[l], e = bl.tile_level_off[l + 1];
+    if (e > b)
+      hipLaunchKernelGGL(kern, dim3(e - b, groups), dim3(64 * W), C::lds_bytes, s, views,
+                         bl.tiles, bl.tile_view, b, bl.active);
+  }
+}
+
+template <int MM, int W>
+void launch_q2_tiles_impl(const Plan &pl, const SweepView &v, hipStream_t s) {
+  using C = Q2TileCfg<MM, W>;
+  auto kern = k_sweep_tile_q2<MM, W>;
+  static bool attr_set = false;
+  if (!attr_set) {
+    OU_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, int(C::lds_bytes)));
+    attr_set = true;
+  }
+  const unsigned groups = unsigned(pl.tile_groups);
+  for (size_t l = 0; l + 1 < pl.tile_level_off.size(); ++l) {
+    const int b = pl.tile_level_off[l], e = pl.tile_level_off[l + 1];
+    if (e > b)
+      hipLaunchKernelGGL(kern, dim3(e - b, groups), dim3(64 * W), C::lds_bytes, s, v,
+                         pl.d_tiles, b);
+  }
+}
+
 // The single-plan and batched launchers of one instantiation.
 struct TileKernels {
   TileLaunch single;
   BatchTileLaunch batch;
 };
+
+template <int MM, int W>
+const TileKernels *q2_kernels_of() {
+  static const TileKernels k{launch_q2_tiles_impl<MM, W>, launch_q2_batch_tiles_impl<MM, W>};
+  return &k;
+}
+
+// 3D Q2: FP32 on the matrix cores, L <= 1 (1 or 4 moments), 3 or 4 waves
+// (the D2M accumulators of 1728 (cell, dof) columns need 192 lanes or more).
+template <typename T>
+const TileKernels *pick_tile_q2(int mm, int waves) {
+  if constexpr (std::is_same<T, float>::value) {
+    if (waves == 3) return mm == 1 ? q2_kernels_of<1, 3>() : mm == 4 ? q2_kernels_of<4, 3>() : nullptr;
+    if (waves == 4) return mm == 1 ? q2_kernels_of<1, 4>() : mm == 4 ? q2_kernels_of<4, 4>() : nullptr;
+  }
+  return nullptr;
+}
 
 template <typename T, int DIM, int P, int MM, int W>
 const TileKernels *tile_kernels_of() {
@@ -478,12 +540,21 @@ const TileKernels *pick_tile(int dim, int p, int mm, int waves) {
     if (waves == 1) return pick_tile_mm<T, 3, 1, 1>(mm);
     if (waves == 2) return pick_tile_mm<T, 3, 1, 2>(mm);
     if (waves == 4) return pick_tile_mm<T, 3, 1, 4>(mm);
+  } else if (dim == 3 && p == 2) {
+    return pick_tile_q2<T>(mm, waves);
   }
   return nullptr;
 }
 
 template <typename T>
 size_t tile_lds(int dim, int p, int mm, int waves) {
+  if (dim == 3 && p == 2)  // Q2TileCfg: psi exchange, staged source, inflow, level tables
+    return size_t(4 * waves) * mm * sizeof(double) +
+           2 * size_t(4 * waves) * q2::kWidest * q2::kRow * sizeof(float) +
+           size_t(64) * q2::kN * mm * sizeof(float) +
+           size_t(4 * waves) * 3 * 16 * q2::kF * sizeof(float) +
+           size_t(4 * waves) * (mm + 4) * sizeof(float) + 16 * q2::kLevels +
+           q2::kWidest * q2::kLevels + 8;
   const int P1 = p + 1, N = dim == 2 ? P1 * P1 : P1 * P1 * P1;
   const int NP = sizeof(T) == 4 ? (N + 3) & ~3 : (N + 1) & ~1;
   const int dpw = dim == 2 ? TileGeom<2>::DPW : TileGeom<3>::DPW;
@@ -502,7 +573,7 @@ bool tile_sweep_supported(int dim, int p, int mm, int precision, int waves, int 
 }
 
 int tile_waves_max(int dim, int p, int mm, int precision, int max_lds) {
-  for (int w : {4, 2, 1})
+  for (int w : {4, 3, 2, 1})
     if (tile_sweep_supported(dim, p, mm, precision, w, max_lds)) return w;
   return 0;
 }

@@ -183,12 +183,12 @@ def test_quadrant_without_directions(oracle):
 
 
 def test_plans_without_tiles_refuse_the_mode():
-    """3D Q2+, 2D Q3+ and tiles that mix materials have no tile sweep: they
-    start in level launches and refuse OU_SWEEP_TILES."""
+    """3D Q2 in FP64, 2D Q3+ and tiles that mix materials have no tile sweep:
+    they start in level launches and refuse OU_SWEEP_TILES."""
     for spec in (P.hetero_2d(n=16, block=4, n_polar=4, n_azim=8, L=1),   # 4x4 material blocks
                  P.hetero_2d(n=8, block=8, n_polar=4, n_azim=8, L=1, p=3),
                  P.batch_3d(n=4, block=2, quad=ou.Quadrature(ou.LS, 3, sn=4, L_max=0)),
-                 P.headline_3d(n=4, block=4, n_polar=4, n_azim=8, L=1)):  # 3D Q2
+                 P.headline_3d(n=4, block=4, n_polar=4, n_azim=8, L=1)):  # 3D Q2, FP64
         op = spec.operator(ou.FP64)
         assert op.sweep_mode == ou.SWEEP_LEVELS
         with pytest.raises(ou.OuError) as e:
