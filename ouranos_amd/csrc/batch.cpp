@@ -630,7 +630,7 @@ std::vector<TuneEntry> Batch::tune(int reps) {
     }
     if (bl_.tiles_ok) {
       set_sweep_mode(OU_SWEEP_TILES);
-      for (int w : {1, 2, 4}) {
+      for (int w : {1, 2, 3, 4}) {  // 3: the 3D Q2 tile sweep's 12-direction rounds
         if (!tile_sweep_supported(p0.dim, p0.p, p0.mm, p0.precision, w, p0.max_lds)) continue;
         for (int g = min_g; g <= max_g; ++g) time_shape(OU_SWEEP_TILES, w, g);
       }

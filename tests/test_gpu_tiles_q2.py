@@ -174,6 +174,37 @@ def test_q2_solvers_in_tile_mode():
     op.close()
 
 
+def test_q2_batch_tile_mode():
+    """A batch of uneven Q2 members in the merged tile levels: within PAIR_TOL
+    of the batch's level launches, the same bits over graph replays and as each
+    member's own tile-mode Apply; the autotuner times the Q2 tile shapes."""
+    specs = [P.headline_3d(n=n, block=4, seed=s, n_polar=4, n_azim=8, L=1)
+             for n, s in ((4, 1), (8, 2), (12, 3))]
+    ops = [s.operator(ou.FP32) for s in specs]
+    batch = ou.Batch(ops, waves=12, dir_groups=1)
+    xs = [P.seeded_vector(s.size, 900 + i) for i, s in enumerate(specs)]
+    lev = batch.apply(xs)
+    batch.set_sweep_mode(ou.SWEEP_TILES)
+    batch.set_shape(3, 1)
+    first = None
+    for rep in range(3):  # eager capture, then graph replays
+        ys = batch.apply(xs)
+        for i, (y, y_lev) in enumerate(zip(ys, lev)):
+            assert rel_err(xs[i] - y, xs[i] - y_lev) < PAIR_TOL, (rep, i)
+        if first is None:
+            first = ys
+        else:
+            assert all(np.array_equal(a, b) for a, b in zip(ys, first)), rep
+    for op, x, y in zip(ops, xs, first):
+        op.sweep_mode = ou.SWEEP_TILES
+        assert op.sweep_shape == (3, 1)
+        assert np.array_equal(op.Apply(x), y)
+    table = batch.tune(reps=1)
+    assert {(e["mode"], e["waves"]) for e in table if e["mode"] == ou.SWEEP_TILES} == \
+        {(ou.SWEEP_TILES, 3), (ou.SWEEP_TILES, 4)}
+    batch.close()
+
+
 def test_q2_full_headline_size_tiles():
     """128^3 Q2, GLC 12x24 (36 directions per octant, three rounds), L=1 in
     the tile sweep: within PAIR_TOL of the FP32 level launches, and linear."""
