@@ -174,6 +174,41 @@ def test_q2_solvers_in_tile_mode():
     op.close()
 
 
+class _Subset:
+    """Some directions of a quadrature (the operator takes any omega / M2D / D2M)."""
+
+    def __init__(self, q, keep):
+        self.omega = np.ascontiguousarray(q.omega[keep])
+        self.M2D = np.ascontiguousarray(q.M2D[keep])
+        self.D2M = np.ascontiguousarray(q.D2M[:, keep])
+        self.mom_l = q.mom_l
+        self.most_normal = np.ascontiguousarray(q.most_normal[:, keep])
+        self.weight_sum = q.weight_sum
+        self.n_dir = int(self.omega.shape[0])
+        self.n_mom = q.n_mom
+
+
+def test_q2_octant_without_directions(oracle):
+    """No direction in the (-,-,-) octant: no tiles for it, its partial-moment
+    slots are skipped whatever a previous shape left there, and the tile sweep
+    stays within the FP32 bound of the oracle across shape changes."""
+    spec = P.headline_3d(n=8, block=4, n_polar=8, n_azim=16, L=1)
+    q = spec.quad
+    keep = ~((q.omega[:, 0] < 0) & (q.omega[:, 1] < 0) & (q.omega[:, 2] < 0))
+    spec.quad = _Subset(q, keep)
+    x = P.seeded_vector(spec.size, 19)
+    ref = oracle_problem(oracle, spec).apply(x)
+    op = _tiles(spec)
+    for mode, shape in [(ou.SWEEP_TILES, (3, 1)), (ou.SWEEP_TILES, (4, 3)),
+                        (ou.SWEEP_LEVELS, (12, 2)), (ou.SWEEP_TILES, (3, 1))]:
+        op.sweep_mode = mode
+        op.sweep_shape = shape
+        err = rel_err(x - op.Apply(x), x - ref)
+        print(f"[gpu] q2 octant without directions {mode} {shape}: {err:.2e}", flush=True)
+        assert err < FP32_TOL, (mode, shape, err)
+    op.close()
+
+
 def test_q2_batch_tile_mode():
     """A batch of uneven Q2 members in the merged tile levels: within PAIR_TOL
     of the batch's level launches, the same bits over graph replays and as each
