@@ -336,7 +336,13 @@ RefinedResult solve_refined(RadiativeTransfer &rt64, RadiativeTransfer &rt32,
     fail(OU_ERR_SIZE, "FP64 and FP32 plans describe different problems");
   if (rt64.plan().precision != OU_FP64 || rt32.plan().precision != OU_FP32)
     fail(OU_ERR_ARG, "refinement needs an FP64 plan and an FP32 plan");
-  hipStream_t s = rt64.plan().stream;
+  // Each plan's work runs on its own stream: the FP64 residuals on the FP64
+  // plan's, the inner FP32 solve on the FP32 plan's.  A plan's sweep forks
+  // its octant halves onto the plan's own side streams, which were created
+  // with it; another plan's stream may share a hardware queue with one of
+  // them, which would serialise the halves.  The host synchronises between
+  // the phases (the residual norm, and the inner solve's end).
+  hipStream_t s = rt64.plan().stream, s32 = rt32.plan().stream;
   const long n = rt64.DomainSize();
   double *b = dev_doubles(n), *r = nullptr, *d = nullptr, *scr = nullptr, *dt = nullptr;
   struct Free {
@@ -368,6 +374,9 @@ RefinedResult solve_refined(RadiativeTransfer &rt64, RadiativeTransfer &rt32,
     return res;
   }
   double fp64_secs = 0.0;
+  // One Krylov workspace for every inner solve (60 GB at the headline size).
+  if (restart > 31) fail(OU_ERR_ARG, "restart length must be <= 31");
+  GmresWorkspace ws(n, restart > 0 ? restart : 30, prec != nullptr);
   // A zero initial guess has residual b: the first FP64 application is skipped.
   bool zero_guess = norm(d_phi) == 0.0;
   for (int cycle = 1;; ++cycle) {
@@ -394,13 +403,14 @@ RefinedResult solve_refined(RadiativeTransfer &rt64, RadiativeTransfer &rt32,
       break;
     }
     if (res.total.iterations >= max_it) break;
-    vec_fill(n, 0.0, d, s);
+    vec_fill(n, 0.0, d, s32);
     // An inner solve that stagnates above inner_tol (the FP32 operator's own
     // accuracy can stop it) hands over to a new FP64 residual after four
-    // restart cycles.
+    // restart cycles.  (r is complete: norm() synchronised its stream.)
     const int inner_max = std::min(max_it - res.total.iterations, 4 * std::max(restart, 1));
     const SolveResult in =
-        solve_gmres(rt32, r, d, restart, inner_tol, inner_max, prec, verbose, s);
+        solve_gmres(rt32, r, d, restart, inner_tol, inner_max, prec, verbose, s32, nullptr, &ws);
+    OU_HIP(hipStreamSynchronize(s32));
     res.total.iterations += in.iterations;
     vec_axpby(n, 1.0, d, 1.0, d_phi, s);
     if (in.iterations == 0) break;

@@ -89,9 +89,32 @@ struct ArnoldiObserver {
   virtual void observe(int iteration, const double *d_z, const double *d_az, hipStream_t s) = 0;
 };
 
+// The Krylov basis of solve_gmres: restart + 1 vectors V and, for the
+// flexible (preconditioned) form, restart vectors Z.  At the headline size
+// that is 60 GB; a caller that runs several solves of one size (the
+// refinement cycles) allocates it once instead of once per solve.
+class GmresWorkspace {
+ public:
+  GmresWorkspace(long n, int restart, bool flexible);
+  ~GmresWorkspace();
+  GmresWorkspace(const GmresWorkspace &) = delete;
+  GmresWorkspace &operator=(const GmresWorkspace &) = delete;
+  bool fits(long n, int restart, bool flexible) const {
+    return n == n_ && restart == restart_ && (flexible ? Z_ != nullptr : true);
+  }
+  double *V() const { return V_; }
+  double *Z() const { return Z_; }
+
+ private:
+  long n_;
+  int restart_;
+  double *V_ = nullptr, *Z_ = nullptr;
+};
+
 SolveResult solve_gmres(Operator &op, const double *d_b, double *d_x, int restart,
                         double tol, int max_it, Preconditioner *prec, bool verbose,
-                        hipStream_t s, ArnoldiObserver *observer = nullptr);
+                        hipStream_t s, ArnoldiObserver *observer = nullptr,
+                        GmresWorkspace *ws = nullptr);
 SolveResult solve_bicgstab(Operator &op, const double *d_b, double *d_x, double tol,
                            int max_it, bool verbose, hipStream_t s);
 // Source iteration on A x = b: x <- x - A x + b (= D L^-1 (M Sigma x + the

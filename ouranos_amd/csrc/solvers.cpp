@@ -131,15 +131,40 @@ SolveResult solve_si_rhs(Operator &op, const double *d_b, double *d_x, double to
   return res;
 }
 
+GmresWorkspace::GmresWorkspace(long n, int restart, bool flexible) : n_(n), restart_(restart) {
+  const size_t nv = size_t(n) * (restart + 1), nz = flexible ? size_t(n) * restart : 0;
+  if (hipMalloc(&V_, nv * sizeof(double)) != hipSuccess ||
+      (nz && hipMalloc(&Z_, nz * sizeof(double)) != hipSuccess)) {
+    (void)hipGetLastError();
+    if (V_) (void)hipFree(V_);
+    V_ = nullptr;
+    fail(OU_ERR_NOMEM, "solver workspace allocation failed");
+  }
+}
+
+GmresWorkspace::~GmresWorkspace() {
+  if (V_) (void)hipFree(V_);
+  if (Z_) (void)hipFree(Z_);
+}
+
 SolveResult solve_gmres(Operator &op, const double *d_b, double *d_x, int restart,
                         double tol, int max_it, Preconditioner *prec, bool verbose,
-                        hipStream_t s, ArnoldiObserver *observer) {
+                        hipStream_t s, ArnoldiObserver *observer, GmresWorkspace *ws) {
   const long n = op.DomainSize();
   const int m = restart > 0 ? restart : 30;
   if (m > 31) fail(OU_ERR_ARG, "restart length must be <= 31");
-  DevBuf V(size_t(n) * (m + 1)), hd(size_t(m + 1));
-  std::unique_ptr<DevBuf> Z;
-  if (prec) Z.reset(new DevBuf(size_t(n) * m));
+  // the caller's workspace when it fits, else one for this solve
+  std::unique_ptr<GmresWorkspace> own;
+  if (!ws || !ws->fits(n, m, prec != nullptr)) {
+    own.reset(new GmresWorkspace(n, m, prec != nullptr));
+    ws = own.get();
+  }
+  DevBuf hd(size_t(m + 1));
+  struct Basis {
+    double *p;
+  };
+  const Basis V{ws->V()}, Zb{ws->Z()};
+  const Basis *Z = prec ? &Zb : nullptr;
   Dotter dt(m + 1);
   int reorth = 0;  // Arnoldi steps that needed the second Gram-Schmidt pass
   std::vector<double> H(size_t(m + 1) * m, 0.0), cs(m), sn(m), g(m + 1), y(m), h(m + 1);
