@@ -160,7 +160,7 @@ Plan::~Plan() {
     if (b) (void)hipFree(b);
   for (hipEvent_t e : ev)
     if (e) (void)hipEventDestroy(e);
-  for (hipEvent_t e : {ev_fork, ev_join[0], ev_join[1], ev_join[2]})
+  for (hipEvent_t e : {ev_fork, ev_join[0], ev_join[1], ev_join[2], ev_join[3]})
     if (e) (void)hipEventDestroy(e);
   for (hipStream_t s : side)
     if (s) (void)hipStreamDestroy(s);

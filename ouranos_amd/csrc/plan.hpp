@@ -164,7 +164,7 @@ struct Plan {
   // sweep's stream by events
   hipStream_t side[3] = {nullptr, nullptr, nullptr};
   hipEvent_t ev_fork = nullptr;
-  hipEvent_t ev_join[3] = {nullptr, nullptr, nullptr};
+  hipEvent_t ev_join[4] = {nullptr, nullptr, nullptr, nullptr};  // side[0..2], then stream
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   void *d_ainv = nullptr, *d_S = nullptr, *d_S0 = nullptr;
   float *d_ainv_mfma = nullptr;

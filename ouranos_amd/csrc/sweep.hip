@@ -445,10 +445,17 @@ void launch_levels_impl(const Plan &pl, const SweepView &v, hipStream_t s,
   if (before) OU_HIP(hipEventRecord(before, s));
   // Octant parts on separate streams: the side streams' launches wait for
   // what s holds before the sweep, and s waits for them before what follows.
+  // The parts run on the plan's own streams, created together (so on
+  // distinct hardware queues): on a caller's stream, part 0 moves to the
+  // plan's stream too, since the caller's stream may share a hardware queue
+  // with a side stream and serialise the parts.
   const int ns = pl.sweep_streams;
+  const bool foreign = ns > 1 && s != pl.stream;
+  hipStream_t part0 = foreign ? pl.stream : s;
   if (ns > 1) {
     OU_HIP(hipEventRecord(pl.ev_fork, s));
     for (int k = 0; k < ns - 1; ++k) OU_HIP(hipStreamWaitEvent(pl.side[k], pl.ev_fork, 0));
+    if (foreign) OU_HIP(hipStreamWaitEvent(pl.stream, pl.ev_fork, 0));
   }
   const unsigned groups = unsigned(pl.dir_groups);
   for (int l = 0; l < pl.n_levels; ++l)
@@ -456,12 +463,16 @@ void launch_levels_impl(const Plan &pl, const SweepView &v, hipStream_t s,
       const int b = pl.quarter_begin(l, 4 * k / ns), e = pl.quarter_begin(l, 4 * (k + 1) / ns);
       if (e > b)
         hipLaunchKernelGGL(kern, dim3(e - b, groups), dim3(64 * W), C::lds_bytes,
-                           k == 0 ? s : pl.side[k - 1], v, b);
+                           k == 0 ? part0 : pl.side[k - 1], v, b);
     }
   if (ns > 1) {
     for (int k = 0; k < ns - 1; ++k) {
       OU_HIP(hipEventRecord(pl.ev_join[k], pl.side[k]));
       OU_HIP(hipStreamWaitEvent(s, pl.ev_join[k], 0));
+    }
+    if (foreign) {
+      OU_HIP(hipEventRecord(pl.ev_join[3], pl.stream));
+      OU_HIP(hipStreamWaitEvent(s, pl.ev_join[3], 0));
     }
   }
   if (after) OU_HIP(hipEventRecord(after, s));

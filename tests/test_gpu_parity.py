@@ -224,6 +224,35 @@ def test_octant_parts_on_several_streams_equal_one(name, precision):
         op.sweep_streams = 3
 
 
+def test_apply_on_a_callers_stream_equals_the_plans():
+    """ou_apply_device on a stream the caller created: the octant parts run on
+    the plan's own streams, forked from and joined to the caller's, and the
+    result equals Apply on the plan's stream bit-for-bit, at 2 and 4 parts."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    stream = ctypes.c_void_p()
+    assert hip.hipStreamCreate(ctypes.byref(stream)) == 0
+    try:
+        spec = CASES["headline_3d_q2_L1"]()
+        op = spec.operator(ou.FP32)
+        x = P.seeded_vector(spec.size, 71)
+        d_x, d_y = ou.DeviceBuffer(x.nbytes), ou.DeviceBuffer(x.nbytes)
+        d_x.upload(x)
+        for parts in (2, 4):
+            op.sweep_streams = parts
+            ref = op.Apply(x)
+            assert ou.lib().ou_apply_device(op._plan, d_x.ptr, d_y.ptr, stream) == 0
+            assert hip.hipStreamSynchronize(stream) == 0
+            y = np.empty(spec.size)
+            d_y.download(y)
+            assert np.array_equal(y, ref), parts
+        d_x.free()
+        d_y.free()
+        op.close()
+    finally:
+        hip.hipStreamDestroy(stream)
+
+
 def test_apply_properties_at_full_2d_size():
     """256x256 Q2, 256 directions, L=3: linearity and sigma_s = 0 => identity."""
     spec = P.hetero_2d()
