@@ -398,9 +398,11 @@ Plan *create_plan(const ou_problem_desc *desc) {
     // ordered so that all chunks of one
     // material (every octant uses the same inverse classes) land on the same
     // XCD: workgroup i of a launch runs on XCD i mod 8, each XCD with its own
-    // L2.  Chunks are bucketed by material mod 8 and dealt out so position p
-    // draws from bucket p mod 8 (from the fullest bucket once that one is
-    // empty).
+    // L2.  Chunks are bucketed by material mod 8 and dealt out so that launch
+    // position p draws from bucket p mod 8 (from the fullest bucket once that
+    // one is empty).  Positions count from the start of the launch in the
+    // default two-part split: a level's launch for octants 0-3 (4-7) holds
+    // quarters 0 and 1 (2 and 3), so quarter 1 (3) continues after quarter 0 (2).
     constexpr int kXcds = 8;
     std::vector<Chunk> chunks;
     p.level_chunk_off.assign(p.n_levels + 1, 0);
@@ -417,8 +419,9 @@ Plan *create_plan(const ou_problem_desc *desc) {
             ++n_half;
           }
         std::vector<size_t> next(kXcds, 0);
+        const size_t base = (q & 1) ? chunks.size() - size_t(p.level_quarter[4 * l + q - 1]) : 0;
         for (size_t pos = 0; pos < n_half; ++pos) {
-          int b = int(pos % kXcds);
+          int b = int((base + pos) % kXcds);
           if (next[b] == xb[b].size()) {
             size_t best = 0;
             for (int q = 0; q < kXcds; ++q)
