@@ -233,7 +233,7 @@ def evaluate(net, n_instances, seed0=EVAL_SEED0, n=32, tol=1e-8, restart=30, max
     return rows
 
 
-def gcr(apply_a, b, precond, tol, restart, max_it, x_exact=None):
+def gcr(apply_a, b, precond, tol, restart, max_it, x_exact=None, progress=None, device=None):
     """Flexible GCR (Eisenstat, Elman and Schultz, SIAM J. Numer. Anal. 20
     (1983)) restarted every `restart` steps: step k applies the preconditioner
     to the current residual, z_k = M(r_k, x_k), makes A z_k orthonormal to the
@@ -241,7 +241,12 @@ def gcr(apply_a, b, precond, tol, restart, max_it, x_exact=None):
     residual over the search directions.  Unlike FGMRES the preconditioner
     always sees the true residual, whose exact correction A^-1 r_k = x* - x_k
     is known when the solution is: that is what the ideal preconditioner of
-    `ceiling` uses.  Returns (x, steps, relative residual history)."""
+    `ceiling` uses.  Returns (x, steps, relative residual history).
+    device: a torch device name ("cuda") keeps the search directions there
+    (the headline family's 60 vectors of 28 M values are slow in numpy);
+    apply_a and precond still take and return numpy arrays."""
+    if device is not None:
+        return _gcr_torch(apply_a, b, precond, tol, restart, max_it, progress, device)
     x = np.zeros_like(b)
     r = b.copy()
     bn = float(np.linalg.norm(b))
@@ -270,11 +275,58 @@ def gcr(apply_a, b, precond, tol, restart, max_it, x_exact=None):
             ps, qs = [], []
             r = b - apply_a(x)  # restart from the true residual
         hist.append(float(np.linalg.norm(r)) / bn)
+        if progress and k % 20 == 0:
+            progress(k, hist[-1])
     return x, k, hist
 
 
+def _gcr_torch(apply_a, b, precond, tol, restart, max_it, progress, device):
+    """gcr() with the vectors in torch tensors on `device` (same arithmetic, FP64)."""
+    import torch
+    dev = torch.device(device)
+
+    def T(a):
+        return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(dev)
+
+    def N(t):
+        return t.cpu().numpy()
+
+    bt = T(b)
+    x = torch.zeros_like(bt)
+    r = bt.clone()
+    bn = float(torch.linalg.norm(bt))
+    hist = [1.0]
+    ps, qs = [], []
+    k = 0
+    while k < max_it and hist[-1] >= tol:
+        z = T(precond(N(r), N(x)))
+        w = T(apply_a(N(z)))
+        k += 1
+        for _ in range(2):
+            for p, q in zip(ps, qs):
+                a = float(torch.dot(q, w))
+                w -= a * q
+                z -= a * p
+        nw = float(torch.linalg.norm(w))
+        if not nw > 0.0:
+            break
+        q, p = w / nw, z / nw
+        alpha = float(torch.dot(q, r))
+        x += alpha * p
+        r -= alpha * q
+        ps.append(p)
+        qs.append(q)
+        if len(ps) == restart:
+            ps, qs = [], []
+            r = bt - T(apply_a(N(x)))  # restart from the true residual
+        hist.append(float(torch.linalg.norm(r)) / bn)
+        if progress and k % 20 == 0:
+            progress(k, hist[-1])
+    return N(x), k, hist
+
+
 def ceiling(net, n_instances=3, seed0=EVAL_SEED0, n=32, grid=32, tol=1e-8, restart=30,
-            max_it=600, log=print):
+            max_it=600, log=print, family="thick", device=None):
     """How far the network's correction space can go.  On held-out instances
     of the thick family (FP64 operator), flexible GCR(restart) to `tol` with
       plain    z = r;
@@ -284,11 +336,13 @@ def ceiling(net, n_instances=3, seed0=EVAL_SEED0, n=32, grid=32, tol=1e-8, resta
                GMRES to 1e-12): the correction the network is trained to
                predict, at its exact value, in the same coarse corner space;
       exact    z = A^-1 r (a one-step solve; checks the driver).
-    Returns one record per instance."""
-    quad = ou.Quadrature(ou.LS, 3, sn=8)
+    family "headline" runs the headline family instead (64^3 Q2, GLC 12x24,
+    L=1; see Family), FP64 operator.  Returns one record per instance."""
+    fam = Family(family)
     rows = []
     for i in range(n_instances):
-        spec = instance(seed0 + i, n=n, quad=quad)
+        spec = (instance(seed0 + i, n=n, quad=fam.quad) if family == "thick"
+                else fam.spec(seed0 + i))
         op = spec.operator(ou.FP64)
         rhs = _rhs_of(op)
         x_star, info = op.solve_rhs(rhs, solver=ou.GMRES, tol=1e-12, max_it=4000)
@@ -316,7 +370,9 @@ def ceiling(net, n_instances=3, seed0=EVAL_SEED0, n=32, grid=32, tol=1e-8, resta
         for name, m in (("plain", lambda r, x: r.copy()), ("learned", prec_learned),
                         ("ideal", prec_ideal), ("exact", prec_exact)):
             t0 = time.perf_counter()
-            x, k, hist = gcr(op.Apply, rhs, m, tol, restart, max_it)
+            x, k, hist = gcr(op.Apply, rhs, m, tol, restart, max_it,
+                             progress=lambda k, h: log(f"    {name} step {k}: {h:.2e}"),
+                             device=device)
             diff = float(np.linalg.norm(x[:spec.n_dofs] - x_star[:spec.n_dofs]) /
                          np.linalg.norm(x_star[:spec.n_dofs]))
             row[name] = {"steps": k, "rel_residual": hist[-1], "scalar_flux_diff": diff,
@@ -397,6 +453,9 @@ def main(argv=None):
     c.add_argument("--weights", required=True)
     c.add_argument("--instances", type=int, default=3)
     c.add_argument("--restart", type=int, default=30)
+    c.add_argument("--family", default="thick", choices=["thick", "headline"])
+    c.add_argument("--device", default=None,
+                   help="torch device for the GCR vectors (e.g. cuda); numpy by default")
     c.add_argument("--out", default=None)
     e = sub.add_parser("evaluate")
     e.add_argument("--weights", required=True)
@@ -456,8 +515,14 @@ def main(argv=None):
             json.dump(rep, f, indent=1)
         log("[learned] done")
     elif a.cmd == "ceiling":
+        if a.device:
+            # torch's HIP runtime initialises before the library's (the other
+            # order leaves torch without a device)
+            import torch
+            torch.zeros(1, device=a.device)
         net = load(a.weights)
-        rows = ceiling(net, a.instances, restart=a.restart, log=log)
+        rows = ceiling(net, a.instances, restart=a.restart, log=log, family=a.family,
+                       device=a.device)
         summary = {m: float(np.mean([r[m]["steps"] for r in rows]))
                    for m in ("plain", "learned", "ideal", "exact")}
         log(json.dumps({"mean_steps": summary}))
