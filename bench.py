@@ -38,7 +38,7 @@ METRIC = "sweep DoF-angle updates/s; time-to-1e-8 learned-FGMRES vs plain GMRES"
 HBM_PEAK_GBS = 8000.0
 # host threads that one GPU's share of the box may use
 CPU_SHARE = 16
-WEIGHTS = os.path.join(ROOT, "ouranos_amd", "weights", "unet_c5_s4.npy")
+WEIGHTS = os.path.join(ROOT, "ouranos_amd", "weights", "unet_c5_g64.npy")
 
 
 def log(msg):

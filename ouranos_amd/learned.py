@@ -335,6 +335,8 @@ def ceiling(net, n_instances=3, seed0=EVAL_SEED0, n=32, grid=32, tol=1e-8, resta
       ideal    z = r + P R (A^-1 r - r), with A^-1 r = x* - x exact (x* from
                GMRES to 1e-12): the correction the network is trained to
                predict, at its exact value, in the same coarse corner space;
+      ideal_moments (more than one moment) the same corner correction for
+               every moment, not only the scalar flux: a richer space;
       exact    z = A^-1 r (a one-step solve; checks the driver).
     family "headline" runs the headline family instead (64^3 Q2, GLC 12x24,
     L=1; see Family), FP64 operator.  Returns one record per instance."""
@@ -363,12 +365,35 @@ def ceiling(net, n_instances=3, seed0=EVAL_SEED0, n=32, grid=32, tol=1e-8, resta
             e = x_star - x
             return op.coarse_prolong(grid, (coarse(e) - coarse(r)).astype(np.float32), z=r)
 
+        nd, n_mom = spec.n_dofs, op.n // spec.n_dofs
+
+        def moment_field(v, m):
+            """Moment m of v, placed where the transfers read moment 0."""
+            out = np.zeros(op.n)
+            out[:nd] = v[m * nd:(m + 1) * nd]
+            return out
+
+        def prec_ideal_moments(r, x):
+            # the ideal corner correction of every moment, not only the scalar flux
+            e = x_star - x
+            z = r.copy()
+            for m in range(n_mom):
+                corners = coarse(moment_field(e, m)) - coarse(moment_field(r, m))
+                z[m * nd:(m + 1) * nd] += op.coarse_prolong(grid, corners.astype(np.float32))[:nd]
+            return z
+
         def prec_exact(r, x):
             return x_star - x
 
         row = {"seed": seed0 + i, "reference_gmres_iterations": info["iterations"]}
-        for name, m in (("plain", lambda r, x: r.copy()), ("learned", prec_learned),
-                        ("ideal", prec_ideal), ("exact", prec_exact)):
+        methods = [("plain", lambda r, x: r.copy())]
+        if grid == net.grid:  # the network runs on the grid it was trained for
+            methods.append(("learned", prec_learned))
+        methods.append(("ideal", prec_ideal))
+        if n_mom > 1:
+            methods.append(("ideal_moments", prec_ideal_moments))
+        methods.append(("exact", prec_exact))
+        for name, m in methods:
             t0 = time.perf_counter()
             x, k, hist = gcr(op.Apply, rhs, m, tol, restart, max_it,
                              progress=lambda k, h: log(f"    {name} step {k}: {h:.2e}"),
@@ -454,6 +479,8 @@ def main(argv=None):
     c.add_argument("--instances", type=int, default=3)
     c.add_argument("--restart", type=int, default=30)
     c.add_argument("--family", default="thick", choices=["thick", "headline"])
+    c.add_argument("--grid", type=int, default=32,
+                   help="coarse grid of the ideal corrections (the network runs only on its own)")
     c.add_argument("--device", default=None,
                    help="torch device for the GCR vectors (e.g. cuda); numpy by default")
     c.add_argument("--out", default=None)
@@ -521,10 +548,11 @@ def main(argv=None):
             import torch
             torch.zeros(1, device=a.device)
         net = load(a.weights)
-        rows = ceiling(net, a.instances, restart=a.restart, log=log, family=a.family,
-                       device=a.device)
+        rows = ceiling(net, a.instances, grid=a.grid, restart=a.restart, log=log,
+                       family=a.family, device=a.device)
         summary = {m: float(np.mean([r[m]["steps"] for r in rows]))
-                   for m in ("plain", "learned", "ideal", "exact")}
+                   for m in ("plain", "learned", "ideal", "ideal_moments", "exact")
+                   if all(m in r for r in rows)}
         log(json.dumps({"mean_steps": summary}))
         if a.out:
             with open(a.out, "w") as f:
