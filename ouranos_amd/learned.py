@@ -41,10 +41,14 @@ class Family:
     headline: 64^3 DG Q2, product GLC 12x24, L=1, 8^3-cell blocks with the
               headline materials (sigma_t log-U[1,500], c U[0.99,0.9999]); on
               the 32^3 coarse grid its coarse cells and blocks have the optical
-              sizes of the 128^3 headline problem's; FP32 sweeps."""
+              sizes of the 128^3 headline problem's; FP32 sweeps;
+    headline128: the headline problem's own generator and size (128^3 DG Q2,
+              GLC 12x24, L=1, 16^3-cell blocks) at other seeds; FP32 sweeps."""
+
+    NAMES = ("thick", "headline", "headline128")
 
     def __init__(self, name):
-        if name not in ("thick", "headline"):
+        if name not in self.NAMES:
             raise ValueError(f"unknown family {name}")
         self.name = name
         self.precision = ou.FP64 if name == "thick" else ou.FP32
@@ -54,6 +58,8 @@ class Family:
     def spec(self, seed):
         if self.name == "thick":
             return P.thick_3d(n=32, block=4, seed=seed, quad=self.quad)
+        if self.name == "headline128":
+            return P.headline_3d(n=128, block=16, seed=seed, quad=self.quad)
         return P.headline_3d(n=64, block=8, seed=seed, quad=self.quad)
 
 
@@ -464,7 +470,7 @@ def main(argv=None):
     t.add_argument("--seed", type=int, default=0)
     t.add_argument("--out", required=True)
     t.add_argument("--mode", type=int, default=0)
-    t.add_argument("--family", default="thick", choices=["thick", "headline"])
+    t.add_argument("--family", default="thick", choices=list(Family.NAMES))
     t.add_argument("--sources", default="si,krylov")
     t.add_argument("--init", default=None, help="start from these weights")
     t.add_argument("--seed-offset", type=int, default=0,
@@ -478,7 +484,7 @@ def main(argv=None):
     c.add_argument("--weights", required=True)
     c.add_argument("--instances", type=int, default=3)
     c.add_argument("--restart", type=int, default=30)
-    c.add_argument("--family", default="thick", choices=["thick", "headline"])
+    c.add_argument("--family", default="thick", choices=list(Family.NAMES))
     c.add_argument("--grid", type=int, default=32,
                    help="coarse grid of the ideal corrections (the network runs only on its own)")
     c.add_argument("--device", default=None,

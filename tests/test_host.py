@@ -261,3 +261,24 @@ def test_plan_rejects_meshes_past_32_bit_indices(dim, p, n, n_dir):
     assert r == -4, ou.lib().ou_last_error()
     assert b"32-bit" in ou.lib().ou_last_error()
     assert not plan.value
+
+
+def test_training_families_are_held_out_from_the_headline():
+    """The learned preconditioner's training and validation instances come
+    from seeds far from the headline problem's own (seed 5), so the headline
+    solve is always on an instance the network never saw.  headline128 is the
+    headline generator at the headline's own size."""
+    from ouranos_amd import learned as Lm
+    from ouranos_amd import problems as P
+    head = P.headline_3d()
+    fam = Lm.Family("headline128")
+    s = fam.spec(Lm.TRAIN_SEED0 + 6000)
+    assert s.sigma_t.shape == head.sigma_t.shape == (512,)
+    assert s.updates_per_sweep() == head.updates_per_sweep()
+    assert not np.array_equal(s.sigma_t, head.sigma_t)
+    assert fam.spec(Lm.TRAIN_SEED0 + 6000).sigma_t.tolist() == s.sigma_t.tolist()
+    assert min(Lm.TRAIN_SEED0, Lm.EVAL_SEED0) > 5
+    small = Lm.Family("headline").spec(Lm.EVAL_SEED0)
+    assert small.sigma_t.shape == (512,) and 8 * small.updates_per_sweep() == s.updates_per_sweep()
+    with pytest.raises(ValueError):
+        Lm.Family("nope")
