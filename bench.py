@@ -38,7 +38,7 @@ METRIC = "sweep DoF-angle updates/s; time-to-1e-8 learned-FGMRES vs plain GMRES"
 HBM_PEAK_GBS = 8000.0
 # host threads that one GPU's share of the box may use
 CPU_SHARE = 16
-WEIGHTS = os.path.join(ROOT, "ouranos_amd", "weights", "unet_c5_g64.npy")
+WEIGHTS = os.path.join(ROOT, "ouranos_amd", "weights", "unet_c5_h128.npy")
 
 
 def log(msg):
@@ -168,8 +168,9 @@ def time_to_tolerance(spec, op32, tol):
                   "wall_seconds": wall_p, "iterations": info_p["iterations"],
                   "fp64_rel_residual": info_p["rel_residual"],
                   "converged": bool(info_p["converged"])},
-        "learned": {"method": ("FGMRES(16) on the FP32 sweep (FP64 moments), learned right "
-                               "preconditioner (U-Net, BF16), FP64 refinement"),
+        "learned": {"method": ("FGMRES(16) on the FP32 sweep (FP64 moments), FP32 Krylov basis "
+                               "(FP64 inner products), learned right preconditioner (U-Net, "
+                               "BF16, 64^3 coarse grid), FP64 refinement"),
                     "weights": os.path.relpath(WEIGHTS, ROOT),
                     "seconds": info_l["seconds"], "wall_seconds": wall_l,
                     "iterations": info_l["iterations"],

@@ -303,8 +303,10 @@ int ou_solve_learned(ou_plan *plan, ou_unet *net, const ou_solver_opts *opts, do
 
 /* Mixed precision: FP64 residuals on plan64, inner FGMRES(restart) solves of
  * the correction on the FP32 plan32 (same problem) to inner_tol relative,
- * right-preconditioned by net (NULL = plain GMRES).  opts->tol is the FP64
- * relative residual to reach; opts->max_it bounds the FP32 applications. */
+ * right-preconditioned by net (NULL = plain GMRES).  The inner solves keep
+ * their Krylov basis in FP32 (FP64 inner products and iterate).  opts->tol
+ * is the FP64 relative residual to reach; opts->max_it bounds the FP32
+ * applications. */
 int ou_solve_refined(ou_plan *plan64, ou_plan *plan32, ou_unet *net,
                      const ou_solver_opts *opts, double inner_tol, double *phi,
                      ou_solve_info *info);

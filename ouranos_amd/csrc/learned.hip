@@ -374,9 +374,12 @@ RefinedResult solve_refined(RadiativeTransfer &rt64, RadiativeTransfer &rt32,
     return res;
   }
   double fp64_secs = 0.0;
-  // One Krylov workspace for every inner solve (60 GB at the headline size).
+  // One Krylov workspace for every inner solve.  The inner solve keeps its
+  // basis in FP32 (its tolerance is far above FP32 rounding; every pass over
+  // the basis then reads half the bytes): 33 GB at the headline size with
+  // restart 16 and a preconditioner.
   if (restart > 31) fail(OU_ERR_ARG, "restart length must be <= 31");
-  GmresWorkspace ws(n, restart > 0 ? restart : 30, prec != nullptr);
+  GmresWorkspace ws(n, restart > 0 ? restart : 30, prec != nullptr, true);
   // A zero initial guess has residual b: the first FP64 application is skipped.
   bool zero_guess = norm(d_phi) == 0.0;
   for (int cycle = 1;; ++cycle) {
@@ -408,8 +411,8 @@ RefinedResult solve_refined(RadiativeTransfer &rt64, RadiativeTransfer &rt32,
     // accuracy can stop it) hands over to a new FP64 residual after four
     // restart cycles.  (r is complete: norm() synchronised its stream.)
     const int inner_max = std::min(max_it - res.total.iterations, 4 * std::max(restart, 1));
-    const SolveResult in =
-        solve_gmres(rt32, r, d, restart, inner_tol, inner_max, prec, verbose, s32, nullptr, &ws);
+    const SolveResult in = solve_gmres_f32_basis(rt32, r, d, restart, inner_tol, inner_max, prec,
+                                                 verbose, s32, &ws);
     OU_HIP(hipStreamSynchronize(s32));
     res.total.iterations += in.iterations;
     vec_axpby(n, 1.0, d, 1.0, d_phi, s);

@@ -92,29 +92,46 @@ struct ArnoldiObserver {
 // The Krylov basis of solve_gmres: restart + 1 vectors V and, for the
 // flexible (preconditioned) form, restart vectors Z.  At the headline size
 // that is 60 GB; a caller that runs several solves of one size (the
-// refinement cycles) allocates it once instead of once per solve.
+// refinement cycles) allocates it once instead of once per solve.  With
+// `f32_basis` the vectors are FP32 (for solve_gmres_f32_basis), plus three
+// FP64 work vectors.
 class GmresWorkspace {
  public:
-  GmresWorkspace(long n, int restart, bool flexible);
+  GmresWorkspace(long n, int restart, bool flexible, bool f32_basis = false);
   ~GmresWorkspace();
   GmresWorkspace(const GmresWorkspace &) = delete;
   GmresWorkspace &operator=(const GmresWorkspace &) = delete;
-  bool fits(long n, int restart, bool flexible) const {
-    return n == n_ && restart == restart_ && (flexible ? Z_ != nullptr : true);
+  bool fits(long n, int restart, bool flexible, bool f32_basis = false) const {
+    return n == n_ && restart == restart_ && f32_basis == f32_ &&
+           (flexible ? Z_ != nullptr : true);
   }
-  double *V() const { return V_; }
-  double *Z() const { return Z_; }
+  double *V() const { return f32_ ? nullptr : static_cast<double *>(V_); }
+  double *Z() const { return f32_ ? nullptr : static_cast<double *>(Z_); }
+  float *Vf() const { return f32_ ? static_cast<float *>(V_) : nullptr; }
+  float *Zf() const { return f32_ ? static_cast<float *>(Z_) : nullptr; }
+  double *work() const { return work_; }  // 3 n doubles (FP32 basis only)
 
  private:
   long n_;
   int restart_;
-  double *V_ = nullptr, *Z_ = nullptr;
+  bool f32_;
+  void *V_ = nullptr, *Z_ = nullptr;
+  double *work_ = nullptr;
 };
 
 SolveResult solve_gmres(Operator &op, const double *d_b, double *d_x, int restart,
                         double tol, int max_it, Preconditioner *prec, bool verbose,
                         hipStream_t s, ArnoldiObserver *observer = nullptr,
                         GmresWorkspace *ws = nullptr);
+// The same restarted (flexible) GMRES with its Krylov basis stored in FP32:
+// every Gram-Schmidt pass reads half the bytes.  The operator's input and
+// output, the vector being orthogonalised, every inner product and the
+// iterate stay FP64; the stored basis carries a relative rounding of 2^-24.
+// It is for inner solves whose tolerance is far above that (the FP32-operator
+// solve of the refinement), not for an FP64 solve to 1e-12.
+SolveResult solve_gmres_f32_basis(Operator &op, const double *d_b, double *d_x, int restart,
+                                  double tol, int max_it, Preconditioner *prec, bool verbose,
+                                  hipStream_t s, GmresWorkspace *ws = nullptr);
 SolveResult solve_bicgstab(Operator &op, const double *d_b, double *d_x, double tol,
                            int max_it, bool verbose, hipStream_t s);
 // Source iteration on A x = b: x <- x - A x + b (= D L^-1 (M Sigma x + the

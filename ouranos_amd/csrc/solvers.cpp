@@ -1,10 +1,13 @@
 // Solver drivers around the transport operator (src/ouranos.cc:43-209):
 // source iteration, restarted GMRES, BiCGStab and flexible GMRES.  Vectors
 // live in HBM; the Arnoldi process, every inner product and the residual test
-// are FP64.  Small dense work (Hessenberg, Givens rotations) is on the host.
+// are FP64 (one GMRES form stores its Krylov basis in FP32, for inner solves).
+// Small dense work (Hessenberg, Givens rotations) is on the host.
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <memory>
 #include <vector>
 
 #include "radiative_transfer.hpp"
@@ -45,6 +48,12 @@ struct Dotter {
     vec_axpy_dots(n, m, V, ldv, h, w, dots, scratch.p, out.p, s);
     return fetch(dots ? m + 1 : 1, s);
   }
+  // The FP32-basis pass: [w -= V h,] then <V_i, w> (i < m, when `dots`) and <w, w>.
+  const std::vector<double> &gs_f32(long n, int m, const float *V, long ldv, const double *h,
+                                    double *w, bool update, bool dots, hipStream_t s) {
+    vec_gs_f32(n, m, V, ldv, h, w, update, dots, scratch.p, out.p, s);
+    return fetch(dots ? m + 1 : 1, s);
+  }
   const std::vector<double> &fetch(int m, hipStream_t s) {
     OU_HIP(hipMemcpyAsync(host.data(), out.p, size_t(m) * 8, hipMemcpyDeviceToHost, s));
     OU_HIP(hipStreamSynchronize(s));
@@ -66,6 +75,47 @@ void check_finite(double v, const char *what, int it) {
     fail(OU_ERR_NONFINITE, buf);
   }
 }
+
+// The small least-squares problem of restarted GMRES on the host: the
+// Hessenberg columns, reduced by Givens rotations as they arrive, and the
+// rotated right-hand side g (|g[j+1]| is the residual norm after column j).
+struct Hessenberg {
+  int m;
+  std::vector<double> H, cs, sn, g, y;
+  explicit Hessenberg(int m_) : m(m_), H(size_t(m_ + 1) * m_, 0.0), cs(m_), sn(m_), g(m_ + 1), y(m_) {}
+  void start(double beta) {
+    std::fill(g.begin(), g.end(), 0.0);
+    g[0] = beta;
+  }
+  // Column j: h[0..j] and the subdiagonal hn; returns |g[j+1]|.
+  double add_column(int j, const double *h, double hn) {
+    for (int i = 0; i <= j; ++i) H[size_t(i) * m + j] = h[i];
+    H[size_t(j + 1) * m + j] = hn;
+    for (int i = 0; i < j; ++i) {
+      const double a = H[size_t(i) * m + j], b = H[size_t(i + 1) * m + j];
+      H[size_t(i) * m + j] = cs[i] * a + sn[i] * b;
+      H[size_t(i + 1) * m + j] = -sn[i] * a + cs[i] * b;
+    }
+    const double a = H[size_t(j) * m + j], b = H[size_t(j + 1) * m + j];
+    const double r = std::hypot(a, b);
+    cs[j] = a / r;
+    sn[j] = b / r;
+    H[size_t(j) * m + j] = r;
+    H[size_t(j + 1) * m + j] = 0.0;
+    g[j + 1] = -sn[j] * g[j];
+    g[j] = cs[j] * g[j];
+    return std::fabs(g[j + 1]);
+  }
+  // y solving the leading j x j triangle against g.
+  const std::vector<double> &solve(int j) {
+    for (int i = j - 1; i >= 0; --i) {
+      double acc = g[i];
+      for (int k = i + 1; k < j; ++k) acc -= H[size_t(i) * m + k] * y[k];
+      y[i] = acc / H[size_t(i) * m + i];
+    }
+    return y;
+  }
+};
 
 }  // namespace
 
@@ -131,13 +181,16 @@ SolveResult solve_si_rhs(Operator &op, const double *d_b, double *d_x, double to
   return res;
 }
 
-GmresWorkspace::GmresWorkspace(long n, int restart, bool flexible) : n_(n), restart_(restart) {
+GmresWorkspace::GmresWorkspace(long n, int restart, bool flexible, bool f32_basis)
+    : n_(n), restart_(restart), f32_(f32_basis) {
+  const size_t es = f32_basis ? sizeof(float) : sizeof(double);
   const size_t nv = size_t(n) * (restart + 1), nz = flexible ? size_t(n) * restart : 0;
-  if (hipMalloc(&V_, nv * sizeof(double)) != hipSuccess ||
-      (nz && hipMalloc(&Z_, nz * sizeof(double)) != hipSuccess)) {
+  if (hipMalloc(&V_, nv * es) != hipSuccess || (nz && hipMalloc(&Z_, nz * es) != hipSuccess) ||
+      (f32_basis && hipMalloc(&work_, 3 * size_t(n) * sizeof(double)) != hipSuccess)) {
     (void)hipGetLastError();
     if (V_) (void)hipFree(V_);
-    V_ = nullptr;
+    if (Z_) (void)hipFree(Z_);
+    V_ = Z_ = nullptr;
     fail(OU_ERR_NOMEM, "solver workspace allocation failed");
   }
 }
@@ -145,6 +198,7 @@ GmresWorkspace::GmresWorkspace(long n, int restart, bool flexible) : n_(n), rest
 GmresWorkspace::~GmresWorkspace() {
   if (V_) (void)hipFree(V_);
   if (Z_) (void)hipFree(Z_);
+  if (work_) (void)hipFree(work_);
 }
 
 SolveResult solve_gmres(Operator &op, const double *d_b, double *d_x, int restart,
@@ -167,7 +221,8 @@ SolveResult solve_gmres(Operator &op, const double *d_b, double *d_x, int restar
   const Basis *Z = prec ? &Zb : nullptr;
   Dotter dt(m + 1);
   int reorth = 0;  // Arnoldi steps that needed the second Gram-Schmidt pass
-  std::vector<double> H(size_t(m + 1) * m, 0.0), cs(m), sn(m), g(m + 1), y(m), h(m + 1);
+  Hessenberg hs(m);
+  std::vector<double> h(m + 1);
   SolveResult res;
   const double bnorm = dt.norm(n, d_b, s);
   check_finite(bnorm, "right-hand side norm", 0);
@@ -201,8 +256,7 @@ SolveResult solve_gmres(Operator &op, const double *d_b, double *d_x, int restar
     }
     if (total >= max_it) break;
     vec_scale(n, 1.0 / beta, V.p, s);
-    std::fill(g.begin(), g.end(), 0.0);
-    g[0] = beta;
+    hs.start(beta);
     int j = 0;
     for (; j < m && total < max_it; ++j) {
       double *vj = V.p + size_t(j) * n;
@@ -245,39 +299,133 @@ SolveResult solve_gmres(Operator &op, const double *d_b, double *d_x, int restar
         for (int i = 0; i <= j; ++i) hsum += std::fabs(h[i]);
         check_finite(hsum, "Arnoldi coefficient", total);
       }
-      for (int i = 0; i <= j; ++i) H[size_t(i) * m + j] = h[i];
-      H[size_t(j + 1) * m + j] = hn;
       if (hn > 0.0) vec_scale(n, 1.0 / hn, wj, s);
-      for (int i = 0; i < j; ++i) {
-        const double a = H[size_t(i) * m + j], b = H[size_t(i + 1) * m + j];
-        H[size_t(i) * m + j] = cs[i] * a + sn[i] * b;
-        H[size_t(i + 1) * m + j] = -sn[i] * a + cs[i] * b;
-      }
-      const double a = H[size_t(j) * m + j], b = H[size_t(j + 1) * m + j];
-      const double r = std::hypot(a, b);
-      cs[j] = a / r;
-      sn[j] = b / r;
-      H[size_t(j) * m + j] = r;
-      H[size_t(j + 1) * m + j] = 0.0;
-      g[j + 1] = -sn[j] * g[j];
-      g[j] = cs[j] * g[j];
-      res.rel_residual = std::fabs(g[j + 1]) / bnorm;
+      res.rel_residual = hs.add_column(j, h.data(), hn) / bnorm;
       if (verbose) progress("  it %d: %.3e\n", total, res.rel_residual);
       if (res.rel_residual < tol || hn == 0.0) {
         ++j;
         break;
       }
     }
-    for (int i = j - 1; i >= 0; --i) {
-      double acc = g[i];
-      for (int k = i + 1; k < j; ++k) acc -= H[size_t(i) * m + k] * y[k];
-      y[i] = acc / H[size_t(i) * m + i];
-    }
+    const std::vector<double> &y = hs.solve(j);
     // x += V y  (or Z y): multi_axpy subtracts, so pass -y.
     for (int i = 0; i < j; ++i) h[i] = -y[i];
     OU_HIP(hipStreamSynchronize(s));
     OU_HIP(hipMemcpy(hd.p, h.data(), size_t(j) * 8, hipMemcpyHostToDevice));
     vec_multi_axpy(n, j, prec ? Z->p : V.p, n, hd.p, 1.0, d_x, s);
+    OU_HIP(hipStreamSynchronize(s));
+  }
+  res.iterations = total;
+  OU_HIP(hipStreamSynchronize(s));
+  return res;
+}
+
+SolveResult solve_gmres_f32_basis(Operator &op, const double *d_b, double *d_x, int restart,
+                                  double tol, int max_it, Preconditioner *prec, bool verbose,
+                                  hipStream_t s, GmresWorkspace *ws) {
+  const long n = op.DomainSize();
+  const int m = restart > 0 ? restart : 30;
+  if (m > 31) fail(OU_ERR_ARG, "restart length must be <= 31");
+  std::unique_ptr<GmresWorkspace> own;
+  if (!ws || !ws->fits(n, m, prec != nullptr, true)) {
+    own.reset(new GmresWorkspace(n, m, prec != nullptr, true));
+    ws = own.get();
+  }
+  // FP32 basis V (and Z); FP64 work vectors: v (the current Arnoldi vector
+  // as the operator or preconditioner reads it), w (the vector being
+  // orthogonalised) and z (the preconditioned vector).
+  float *V = ws->Vf(), *Z = ws->Zf();
+  double *v = ws->work(), *w = v + n, *z = v + 2 * n;
+  DevBuf hd(size_t(m + 1));
+  Dotter dt(m + 1);
+  int reorth = 0;
+  Hessenberg hs(m);
+  std::vector<double> h(m + 1);
+  SolveResult res;
+  const double bnorm = dt.norm(n, d_b, s);
+  check_finite(bnorm, "right-hand side norm", 0);
+  if (bnorm == 0.0) {
+    vec_fill(n, 0.0, d_x, s);
+    res.converged = true;
+    OU_HIP(hipStreamSynchronize(s));
+    return res;
+  }
+  int total = 0;
+  bool x_is_zero = dt.norm(n, d_x, s) == 0.0;
+  while (true) {
+    // r = b - A x, in w
+    if (x_is_zero) {
+      vec_copy(n, d_b, w, s);
+      x_is_zero = false;
+    } else {
+      op.Apply(d_x, w, s);
+      vec_lincomb(n, 1.0, d_b, -1.0, w, w, s);
+    }
+    const double beta = dt.norm(n, w, s);
+    check_finite(beta, "residual norm", total);
+    res.rel_residual = beta / bnorm;
+    if (verbose)
+      progress("GMRES (FP32 basis) restart at %d: true residual %.3e (%d reorthogonalised "
+               "steps)\n",
+               total, res.rel_residual, reorth);
+    if (res.rel_residual < tol) {
+      res.converged = true;
+      break;
+    }
+    if (total >= max_it) break;
+    vec_scale_store_f32(n, 1.0 / beta, w, true, V, s);
+    std::swap(v, w);
+    hs.start(beta);
+    int j = 0;
+    for (; j < m && total < max_it; ++j) {
+      if (prec) {
+        prec->apply(v, z, s);
+        op.Apply(z, w, s);
+        vec_scale_store_f32(n, 1.0, z, false, Z + size_t(j) * n, s);
+      } else {
+        op.Apply(v, w, s);
+      }
+      ++total;
+      // Classical Gram-Schmidt with the DGKS test, as in solve_gmres: the
+      // first pass's dot products with |w|^2, the projection fused with the
+      // second pass's dot products, and (when needed) the second projection.
+      double hn = 0.0;
+      {
+        const std::vector<double> &hh = dt.gs_f32(n, j + 1, V, n, nullptr, w, false, true, s);
+        const double wnorm = std::sqrt(std::max(hh[j + 1], 0.0));
+        for (int i = 0; i <= j; ++i) h[i] = hh[i];
+        OU_HIP(hipMemcpy(hd.p, hh.data(), size_t(j + 1) * 8, hipMemcpyHostToDevice));
+        const std::vector<double> &h2 = dt.gs_f32(n, j + 1, V, n, hd.p, w, true, true, s);
+        hn = std::sqrt(std::max(h2[j + 1], 0.0));
+        if (hn < 0.7071067811865476 * wnorm) {
+          for (int i = 0; i <= j; ++i) h[i] += h2[i];
+          OU_HIP(hipMemcpy(hd.p, h2.data(), size_t(j + 1) * 8, hipMemcpyHostToDevice));
+          const std::vector<double> &h3 = dt.gs_f32(n, j + 1, V, n, hd.p, w, true, false, s);
+          hn = std::sqrt(std::max(h3[0], 0.0));
+          ++reorth;
+        }
+      }
+      {
+        double hsum = hn;
+        for (int i = 0; i <= j; ++i) hsum += std::fabs(h[i]);
+        check_finite(hsum, "Arnoldi coefficient", total);
+      }
+      res.rel_residual = hs.add_column(j, h.data(), hn) / bnorm;
+      if (hn > 0.0) {
+        // the next Arnoldi vector: FP64 in v for the next application, FP32 in V
+        vec_scale_store_f32(n, 1.0 / hn, w, true, V + size_t(j + 1) * n, s);
+        std::swap(v, w);
+      }
+      if (verbose) progress("  it %d: %.3e\n", total, res.rel_residual);
+      if (res.rel_residual < tol || hn == 0.0) {
+        ++j;
+        break;
+      }
+    }
+    const std::vector<double> &y = hs.solve(j);
+    OU_HIP(hipStreamSynchronize(s));
+    OU_HIP(hipMemcpy(hd.p, y.data(), size_t(j) * 8, hipMemcpyHostToDevice));
+    vec_multi_axpy_f32(n, j, prec ? Z : V, n, hd.p, 1.0, d_x, s);  // x += V y (or Z y)
     OU_HIP(hipStreamSynchronize(s));
   }
   res.iterations = total;

@@ -22,6 +22,18 @@ void vec_axpy_dots(long n, int m, const double *V, long ldv, const double *h, do
 // w -= sum_i h[i] V_i  (h on the device, m <= 64)
 void vec_multi_axpy(long n, int m, const double *V, long ldv, const double *h,
                     double alpha, double *w, hipStream_t s);
+// The FP32-basis forms, for a Krylov basis stored in FP32 (w, h, x and
+// every sum FP64):
+// when `update`, w -= sum_{i<m} h[i] V_i; then out[i] = <V_i, w> for i < m
+// when `dots`, and out[m or 0] = <w, w>; one pass, m <= 32.
+// scratch: reduce_scratch_doubles(m + 1) doubles.
+void vec_gs_f32(long n, int m, const float *V, long ldv, const double *h, double *w,
+                bool update, bool dots, double *scratch, double *out, hipStream_t s);
+// x += alpha sum_i h[i] V_i  (h on the device, m <= 64)
+void vec_multi_axpy_f32(long n, int m, const float *V, long ldv, const double *h,
+                        double alpha, double *x, hipStream_t s);
+// y = (float)(a x), and x = a x when `scale`
+void vec_scale_store_f32(long n, double a, double *x, bool scale, float *y, hipStream_t s);
 // y = a*x + b*y
 void vec_axpby(long n, double a, const double *x, double b, double *y, hipStream_t s);
 // z = a*x + b*y

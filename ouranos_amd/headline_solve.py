@@ -27,7 +27,7 @@ from ouranos_amd import learned as Lm
 from ouranos_amd import problems as P
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-WEIGHTS = os.path.join(HERE, "weights", "unet_c5_g64.npy")
+WEIGHTS = os.path.join(HERE, "weights", "unet_c5_h128.npy")
 
 
 def log(msg):

@@ -307,3 +307,29 @@ def test_trained_network_gain_over_plain_gmres():
     assert s["learned_iterations_mean"] < 0.5 * s["plain_iterations_mean"]
     assert s["max_scalar_flux_diff"] < 1e-5
     net.close()
+
+
+@pytest.mark.parametrize("restart", [8, 16, 24])
+def test_refinement_with_the_trained_network_matches_fp64_gmres(restart):
+    """Refinement around the preconditioned FP32-operator solve, whose inner
+    FGMRES keeps its Krylov basis in FP32 (restarts 8 and 16 take the fused
+    Gram-Schmidt kernels for up to 8 and 16 vectors, 24 the widest form), on a
+    held-out 32^3 thick instance: FP64 residual below 1e-10, the FP64 GMRES
+    solution to 1e-6, and fewer operator applications than plain GMRES."""
+    from ouranos_amd import learned as Lm
+    spec = Lm.instance(Lm.EVAL_SEED0 + 700)
+    op64, op32 = spec.operator(ou.FP64), spec.operator(ou.FP32)
+    phi, info = op64.solve(solver=ou.GMRES, tol=1e-11, max_it=3000)
+    net = Lm.load(WEIGHTS)
+    phi_r, info_r = ou.solve_refined(op64, op32, net, tol=1e-10, max_it=3000, restart=restart,
+                                     inner_tol=3e-5)
+    print(f"[gpu] refinement, FGMRES({restart}) with the network: {info_r}; "
+          f"plain GMRES {info['iterations']} iterations")
+    nd = spec.n_dofs
+    assert info["converged"] and info_r["converged"] and info_r["rel_residual"] < 1e-10
+    assert info_r["fp64_applications"] >= 2
+    assert rel_err(phi_r[:nd], phi[:nd]) < 1e-6
+    assert info_r["iterations"] < info["iterations"]
+    net.close()
+    op64.close()
+    op32.close()
