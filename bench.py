@@ -11,7 +11,7 @@ flux with FP64 moment accumulation.
 The same line carries the rest of the metric, measured in the same run:
   time_to_tol   time to a 1e-8 relative residual (FP64 residual of the FP64
                 operator) on the same workload: plain FP64 GMRES(30) against
-                the learned right preconditioner in FGMRES(16) on the FP32
+                the learned right preconditioner in FGMRES(24) on the FP32
                 sweep with FP64 refinement; seconds, iterations, residuals,
                 scalar-flux difference and the ratio
   fp64_sweep    the FP64 operator application (the reference's precision):
@@ -138,7 +138,7 @@ def cpu_legs(spec, seconds_budget):
 def time_to_tolerance(spec, op32, tol):
     """Time to a `tol` relative residual, every residual test FP64 on the FP64
     operator: plain FP64 GMRES(30), the reference's Krylov method with the
-    identity preconditioner (ouranos.cc:131-208), against FGMRES(16) on the
+    identity preconditioner (ouranos.cc:131-208), against FGMRES(24) on the
     FP32 operator with the learned right preconditioner and FP64 refinement."""
     import ouranos_amd as ou
     from ouranos_amd import learned as Lm
@@ -153,10 +153,10 @@ def time_to_tolerance(spec, op32, tol):
     del phi_p
     log(f"[bench]   {info_p['iterations']} iterations, {info_p['seconds']:.2f} s")
     net = Lm.load(WEIGHTS)
-    log("[bench] time to tolerance: learned FGMRES(16), FP32 sweep, FP64 refinement")
+    log("[bench] time to tolerance: learned FGMRES(24), FP32 sweep, FP64 refinement")
     t0 = time.perf_counter()
-    phi_l, info_l = ou.solve_refined(op64, op32, net, tol=tol, max_it=3000, restart=16,
-                                     inner_tol=3e-5, raise_on_fail=False)
+    phi_l, info_l = ou.solve_refined(op64, op32, net, tol=tol, max_it=3000, restart=24,
+                                     inner_tol=7e-5, raise_on_fail=False)
     wall_l = time.perf_counter() - t0
     diff = float(np.linalg.norm(phi_l[:nd] - ref) / np.linalg.norm(ref))
     del phi_l
@@ -168,7 +168,8 @@ def time_to_tolerance(spec, op32, tol):
                   "wall_seconds": wall_p, "iterations": info_p["iterations"],
                   "fp64_rel_residual": info_p["rel_residual"],
                   "converged": bool(info_p["converged"])},
-        "learned": {"method": ("FGMRES(16) on the FP32 sweep (FP64 moments), FP32 Krylov basis "
+        "learned": {"method": ("FGMRES(24), inner tol 7e-5, on the FP32 sweep (FP64 moments), "
+                               "FP32 Krylov basis "
                                "(FP64 inner products), learned right preconditioner (U-Net, "
                                "BF16, 64^3 coarse grid), FP64 refinement"),
                     "weights": os.path.relpath(WEIGHTS, ROOT),
